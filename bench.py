@@ -1,0 +1,228 @@
+"""Headline benchmark: Barnes-Hut step at N = 4,194,304, theta = 0.5.
+
+Workload (precision form): circular binary 0.5 + 0.5 at separation 0.2 plus a
+4,194,302-body Keplerian disk (r in [0.3, 2], z ~ N(0, 0.02), m = 1e-7), FP32
+force with Kahan accumulation, FP64 bounding-box extent and cell summaries,
+softening 1e-3, G = 1, dt = 1e-3.  A step is the reference's full stage
+sequence (reset, bounding box, build, centre of mass, sort, forces, update) on
+arrays already resident in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--n N] [--theta T]
+
+Prints one JSON line.  `value` is G interactions/s of the force stage, where
+interactions are the per-particle interaction lists (cells accepted by
+l/D < theta plus leaf bodies) counted on the GPU outside the timed region.
+`roofline` prices the force kernel at 20 flop per interaction against the
+MI355X FP32 vector peak; `cpu_baseline` times the reference's own OpenMP tree
+code (oracle/_ref/ref_harness) on this box's host cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import struct
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (datasheet)
+FLOP_PER_INTERACTION = 20  # GPU Gems 3 convention
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=4194304)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--accuracy-sample", type=int, default=2048,
+                    help="bodies checked against the FP64 direct sum (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=131072,
+                    help="bodies whose reference CPU tree walk is timed (0 = skip the CPU baseline)")
+    ap.add_argument("--no-kahan", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(x, y, z, m, theta, softening, sample):
+    """Times the reference's serial Tree::insert of all bodies and its OpenMP
+    treeInteraction loop over `sample` bodies (oracle/_ref/ref_harness)."""
+    from oracle import oracle as O
+    if not O.have_ref():
+        return None
+    n = len(x)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "bodies.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<q", n))
+            for a in (x, y, z, m):
+                f.write(np.ascontiguousarray(a, np.float32).tobytes())
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        r = subprocess.run([O.REF_HARNESS, "time", fin, repr(theta), repr(softening * O.TO_METERS),
+                            str(sample)], capture_output=True, text=True, env=env, timeout=900)
+    if r.returncode != 0:
+        return {"error": r.stderr[-400:]}
+    rec = json.loads(r.stdout.strip().splitlines()[-1])
+    force_full = rec["force_sample_s"] * n / rec["nsample"]
+    ms = (rec["build_s"] + force_full) * 1e3
+    return {"value": round(ms, 1), "unit": "ms/step (tree build + force)", "cores": rec["threads"],
+            "kind": "reference",
+            "sample": (f"reference Tree::insert of all {n} bodies ({rec['build_s']:.2f} s, serial) + "
+                       f"Interaction::treeInteraction over every {n // rec['nsample']}-th body "
+                       f"({rec['nsample']} bodies, {rec['force_sample_s']:.2f} s on {rec['threads']} threads), "
+                       f"force scaled by n/sample"),
+            "build_s": rec["build_s"], "force_s_extrapolated": round(force_full, 3)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU runs are not part of this build",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import nns_amd
+
+    n = args.n
+    x, y, z, vx, vy, vz, m = nns_amd.initial_conditions("binary_disk", n, args.seed)
+    params = nns_amd.default_params(theta=args.theta, kahan=0 if args.no_kahan else 1)
+    bh = nns_amd.BarnesHut(x, y, z, vx, vy, vz, m, params=params)
+    print(f"# n={n} m={bh.m} theta={args.theta} warmup={args.warmup} steps={args.steps}", file=sys.stderr,
+          flush=True)
+
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        bh.stages()
+    bh.ctx.check()
+    torch.cuda.synchronize()
+    counts0 = bh.count_interactions()
+    torch.cuda.synchronize()
+    i_alg0 = int(counts0.sum().item())
+
+    # timed region: the seven stage calls per step; torch events on the stream the
+    # library launches on (the current stream) bracket each stage
+    K = args.steps
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(8)] for _ in range(K)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    a, kw = bh.arr, bh.kw
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t_start.record(stream)
+    for k in range(K):
+        e = ev[k]
+        e[0].record(stream)
+        kw.resetArrays(a["mutex"], a["x"], a["y"], a["z"], a["mass"], a["count"], a["start"], a["sorted"],
+                       a["child"], a["index"], a["minX"], a["maxX"], a["minY"], a["maxY"], a["minZ"], a["maxZ"],
+                       n, bh.m)
+        e[1].record(stream)
+        kw.computeBoundingBox(a["mutex"], a["x"], a["y"], a["z"], a["minX"], a["maxX"], a["minY"], a["maxY"],
+                              a["minZ"], a["maxZ"], n)
+        e[2].record(stream)
+        kw.buildTree(a["x"], a["y"], a["z"], a["mass"], a["count"], a["start"], a["child"], a["index"],
+                     a["minX"], a["maxX"], a["minY"], a["maxY"], a["minZ"], a["maxZ"], n, bh.m)
+        e[3].record(stream)
+        kw.centreOfMass(a["x"], a["y"], a["z"], a["mass"], a["index"], n)
+        e[4].record(stream)
+        kw.sort(a["count"], a["start"], a["sorted"], a["child"], a["index"], n)
+        e[5].record(stream)
+        kw.computeForces(a["x"], a["y"], a["z"], a["vx"], a["vy"], a["vz"], a["ax"], a["ay"], a["az"], a["mass"],
+                         a["sorted"], a["child"], a["minX"], a["maxX"], n, params.gravity)
+        e[6].record(stream)
+        kw.update(a["x"], a["y"], a["z"], a["vx"], a["vy"], a["vz"], a["ax"], a["ay"], a["az"], n,
+                  params.timestep, params.dampening)
+        e[7].record(stream)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    bh.ctx.check()
+
+    total_ms = t_start.elapsed_time(t_end)
+    stage_names = ("reset", "bbox", "build", "com", "sort", "force", "update")
+    stage_ms = {s: float(np.mean([ev[k][j].elapsed_time(ev[k][j + 1]) for k in range(K)]))
+                for j, s in enumerate(stage_names)}
+    counts1 = bh.count_interactions()
+    torch.cuda.synchronize()
+    i_alg1 = int(counts1.sum().item())
+    i_alg = 0.5 * (i_alg0 + i_alg1)
+    ms_per_step = total_ms / K
+    t_force = stage_ms["force"] * 1e-3
+    ginter = i_alg / t_force / 1e9
+    achieved = FLOP_PER_INTERACTION * i_alg / t_force / 1e12
+
+    accuracy = None
+    if args.accuracy_sample > 0:
+        from oracle import oracle as O
+        # forces at the current positions: the six stages before update
+        xs, ys, zs = bh.host("x"), bh.host("y"), bh.host("z")
+        a2 = bh.arr
+        kw.resetArrays(a2["mutex"], a2["x"], a2["y"], a2["z"], a2["mass"], a2["count"], a2["start"],
+                       a2["sorted"], a2["child"], a2["index"], a2["minX"], a2["maxX"], a2["minY"], a2["maxY"],
+                       a2["minZ"], a2["maxZ"], n, bh.m)
+        kw.computeBoundingBox(a2["mutex"], a2["x"], a2["y"], a2["z"], a2["minX"], a2["maxX"], a2["minY"],
+                              a2["maxY"], a2["minZ"], a2["maxZ"], n)
+        kw.buildTree(a2["x"], a2["y"], a2["z"], a2["mass"], a2["count"], a2["start"], a2["child"], a2["index"],
+                     a2["minX"], a2["maxX"], a2["minY"], a2["maxY"], a2["minZ"], a2["maxZ"], n, bh.m)
+        kw.centreOfMass(a2["x"], a2["y"], a2["z"], a2["mass"], a2["index"], n)
+        kw.sort(a2["count"], a2["start"], a2["sorted"], a2["child"], a2["index"], n)
+        kw.computeForces(a2["x"], a2["y"], a2["z"], a2["vx"], a2["vy"], a2["vz"], a2["ax"], a2["ay"], a2["az"],
+                         a2["mass"], a2["sorted"], a2["child"], a2["minX"], a2["maxX"], n, params.gravity)
+        torch.cuda.synchronize()
+        bh.ctx.check()
+        rng = np.random.default_rng(12345)
+        tgt = np.sort(rng.choice(n, size=min(args.accuracy_sample, n), replace=False)).astype(np.int32)
+        agpu = np.stack([bh.host("ax")[tgt], bh.host("ay")[tgt], bh.host("az")[tgt]], 1).astype(np.float64)
+        t0 = time.perf_counter()
+        ad = O.direct_accel_fast(xs, ys, zs, m, float(np.float32(params.softening) ** 2), tgt)
+        rel = np.linalg.norm(agpu - ad, axis=1) / np.linalg.norm(ad, axis=1)
+        accuracy = {"rms_rel_err_vs_fp64_direct": float(np.sqrt((rel ** 2).mean())),
+                    "max_rel_err": float(rel.max()), "sample": int(len(tgt)),
+                    "direct_sum_s": round(time.perf_counter() - t0, 2)}
+
+    cpu = None
+    if args.cpu_sample > 0:
+        cpu = cpu_baseline(x, y, z, m, args.theta, params.softening, args.cpu_sample)
+        if cpu and "value" in cpu:
+            cpu["gpu_speedup_build_force"] = round(
+                cpu["value"] / (stage_ms["bbox"] + stage_ms["build"] + stage_ms["com"] + stage_ms["sort"]
+                                + stage_ms["force"]), 1)
+
+    out = {
+        "metric": "ms/step & G interactions/s, N=4M theta=0.5; batched & adjoint vs plain",
+        "value": round(ginter, 2),
+        "unit": "G interactions/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "precision form: binary 0.5+0.5 (sep 0.2) + Keplerian disk, "
+                               f"N={n}, theta={args.theta}, eps=1e-3, Kahan={'off' if args.no_kahan else 'on'}",
+                   "n": n, "theta": args.theta, "parallelism": "single GPU"},
+        "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "interactions_per_step": int(i_alg),
+        "wall_s": round(wall, 3),
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "force_kernel", "flop_per_interaction": FLOP_PER_INTERACTION},
+        "cpu_baseline": cpu,
+        "accuracy": accuracy,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,59 @@
+// barnes_hut.h -- C++ simulation object over the C ABI (the reference's
+// BarnesHut class, 3D/CUDA/CUDA_NBody/include/BarnesHut.cuh:18-127).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/nbx.h"
+
+namespace nbx {
+
+struct SimulationParameters {  // include/Constants.h:4-22 plus the engine's knobs
+    bool timeKernels = true;
+    int iterations = 100;
+    int numberOfParticles = 524288;
+    float timestep = 0.001f;
+    float gravity = 1.0f;
+    float dampening = 1.0f;
+    float theta = 0.5f;
+    float softening = 1e-3f;
+    bool kahan = true;
+    std::string model = "binary_disk";  // star_disk | plummer | binary_disk
+    unsigned long long seed = 1;
+    bool copyToHost = false;            // the reference copies x, v back every step
+};
+
+class BarnesHut {
+public:
+    explicit BarnesHut(const SimulationParameters &p);
+    ~BarnesHut();
+    BarnesHut(const BarnesHut &) = delete;
+    BarnesHut &operator=(const BarnesHut &) = delete;
+
+    // One step (src/BarnesHut.cu:195-301); throws std::runtime_error on an engine error.
+    void update(int step);
+    float getSystemSize() const;
+
+    SimulationParameters parameters;
+    int numParticles, numNodes;
+    std::vector<float> h_x, h_y, h_z, h_vx, h_vy, h_vz, h_mass;
+    // per-step stage times in ms: reset, bbox, build, com, sort, forces, update, copy, all
+    std::vector<float> time_resetArrays, time_computeBoundingBox, time_buildTree, time_centreOfMass, time_sort,
+        time_computeForces, time_update, time_copyDeviceToHost, time_all;
+
+private:
+    nbx_context *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    float *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_mass = nullptr, *d_vx = nullptr, *d_vy = nullptr,
+          *d_vz = nullptr, *d_ax = nullptr, *d_ay = nullptr, *d_az = nullptr;
+    int *d_child = nullptr, *d_start = nullptr, *d_sorted = nullptr, *d_count = nullptr, *d_index = nullptr,
+        *d_mutex = nullptr;
+    float *d_min_x = nullptr, *d_max_x = nullptr, *d_min_y = nullptr, *d_max_y = nullptr, *d_min_z = nullptr,
+          *d_max_z = nullptr;
+    void check(int rc, const char *what);
+};
+
+}  // namespace nbx

@@ -1,0 +1,273 @@
+// k_basic.hip -- streaming stages: reset, bounding box, keys, update.
+// All of them are HBM-bound: coalesced grid-stride loops, no LDS except the
+// bounding-box block reduction.
+#include "nbx_internal.h"
+
+namespace nbx {
+
+// resetArraysKernel semantics (3D/CUDA/CUDA_NBody/src/Kernels.cu:21-64).  The
+// eight child slots of a node are written as two 16-byte stores.
+__global__ void __launch_bounds__(256) reset_kernel(
+    int *__restrict__ mutex, float *__restrict__ x, float *__restrict__ y, float *__restrict__ z,
+    float *__restrict__ mass, int *__restrict__ count, int *__restrict__ start, int *__restrict__ sorted,
+    int *__restrict__ child, int *__restrict__ index, float *minX, float *maxX, float *minY, float *maxY,
+    float *minZ, float *maxZ, int n, int m, DeviceState *st, int *arrive)
+{
+    const int stride = blockDim.x * gridDim.x;
+    const int4 neg = make_int4(-1, -1, -1, -1);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        int4 *c4 = reinterpret_cast<int4 *>(child + 8 * (int64_t)i);
+        c4[0] = neg;
+        c4[1] = neg;
+        if (i < n) {
+            count[i] = 1;
+        } else {
+            x[i] = 0.f; y[i] = 0.f; z[i] = 0.f; mass[i] = 0.f;
+            count[i] = 0;
+        }
+        start[i] = -1;
+        sorted[i] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *mutex = 0;
+        *index = n;
+        *minX = 0.f; *maxX = 0.f; *minY = 0.f; *maxY = 0.f; *minZ = 0.f; *maxZ = 0.f;
+        st->err = 0;
+        st->bbox_ticket = 0u;
+        arrive[0] = 0;
+    }
+}
+
+__device__ inline float wave_min(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline float wave_max(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Bounding box: per-block min/max, then the last block to finish (ticket
+// counter) folds the partials and derives the FP64 extent of the key grid:
+// origin (xMin, yMin, zMin) and maxSpan = max |max - min| over the axes
+// (3D/OpenMP/OpenMP_NBody/src/Interaction.cpp:143-147).
+__global__ void __launch_bounds__(256) bbox_kernel(
+    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z, int n,
+    float *__restrict__ part, DeviceState *st, float *minX, float *maxX, float *minY, float *maxY,
+    float *minZ, float *maxZ)
+{
+    __shared__ float red[6][4];
+    __shared__ int s_last, s_bad;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    float a = INFINITY, b = -INFINITY, c = INFINITY, d = -INFINITY, e = INFINITY, f = -INFINITY;
+    int bad = 0;
+    for (int i = blockIdx.x * blockDim.x + tid; i < n; i += blockDim.x * gridDim.x) {
+        float xi = x[i], yi = y[i], zi = z[i];
+        bad |= !(isfinite(xi) && isfinite(yi) && isfinite(zi));
+        a = fminf(a, xi); b = fmaxf(b, xi);
+        c = fminf(c, yi); d = fmaxf(d, yi);
+        e = fminf(e, zi); f = fmaxf(f, zi);
+    }
+    if (tid == 0) s_bad = 0;
+    a = wave_min(a); b = wave_max(b); c = wave_min(c); d = wave_max(d); e = wave_min(e); f = wave_max(f);
+    __syncthreads();
+    if (bad) s_bad = 1;
+    if (lane == 0) {
+        red[0][wave] = a; red[1][wave] = b; red[2][wave] = c;
+        red[3][wave] = d; red[4][wave] = e; red[5][wave] = f;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int nw = blockDim.x >> 6;
+        for (int w = 1; w < nw; ++w) {
+            red[0][0] = fminf(red[0][0], red[0][w]); red[1][0] = fmaxf(red[1][0], red[1][w]);
+            red[2][0] = fminf(red[2][0], red[2][w]); red[3][0] = fmaxf(red[3][0], red[3][w]);
+            red[4][0] = fminf(red[4][0], red[4][w]); red[5][0] = fmaxf(red[5][0], red[5][w]);
+        }
+        for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = red[k][0];
+        if (s_bad) atomicOr(&st->err, kErrNonFinite);
+        __threadfence();
+        unsigned t = atomicAdd(&st->bbox_ticket, 1u);
+        s_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    // last block: fold all partials
+    a = INFINITY; b = -INFINITY; c = INFINITY; d = -INFINITY; e = INFINITY; f = -INFINITY;
+    for (int k = tid; k < (int)gridDim.x; k += blockDim.x) {
+        const float *q = part + 6 * k;
+        a = fminf(a, __hip_atomic_load(q + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        b = fmaxf(b, __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        c = fminf(c, __hip_atomic_load(q + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        d = fmaxf(d, __hip_atomic_load(q + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        e = fminf(e, __hip_atomic_load(q + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        f = fmaxf(f, __hip_atomic_load(q + 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    a = wave_min(a); b = wave_max(b); c = wave_min(c); d = wave_max(d); e = wave_min(e); f = wave_max(f);
+    __syncthreads();
+    if (lane == 0) {
+        red[0][wave] = a; red[1][wave] = b; red[2][wave] = c;
+        red[3][wave] = d; red[4][wave] = e; red[5][wave] = f;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int nw = blockDim.x >> 6;
+        for (int w = 1; w < nw; ++w) {
+            red[0][0] = fminf(red[0][0], red[0][w]); red[1][0] = fmaxf(red[1][0], red[1][w]);
+            red[2][0] = fminf(red[2][0], red[2][w]); red[3][0] = fmaxf(red[3][0], red[3][w]);
+            red[4][0] = fminf(red[4][0], red[4][w]); red[5][0] = fmaxf(red[5][0], red[5][w]);
+        }
+        *minX = red[0][0]; *maxX = red[1][0]; *minY = red[2][0];
+        *maxY = red[3][0]; *minZ = red[4][0]; *maxZ = red[5][0];
+        double xs = fabs((double)red[1][0] - (double)red[0][0]);
+        double ys = fabs((double)red[3][0] - (double)red[2][0]);
+        double zs = fabs((double)red[5][0] - (double)red[4][0]);
+        double sp = xs > ys ? xs : ys;
+        sp = sp < zs ? zs : sp;
+        st->ext[0] = red[0][0]; st->ext[1] = red[2][0]; st->ext[2] = red[4][0]; st->ext[3] = sp;
+        st->bbox_ticket = 0u;
+    }
+}
+
+// Lebesgue2Hilbert state tables (MolecularDynamics/BarnesHutParallel/
+// SpaceFillingCurves/src/Tree.cpp:71-80); the values define the key.
+__constant__ unsigned char c_dir[12][8] = {
+    {8, 10, 3, 3, 4, 5, 4, 5}, {2, 2, 11, 9, 4, 5, 4, 5}, {7, 6, 7, 6, 8, 10, 1, 1},
+    {7, 6, 7, 6, 0, 0, 11, 9}, {0, 8, 1, 11, 6, 8, 6, 11}, {10, 0, 9, 1, 10, 7, 9, 7},
+    {10, 4, 9, 4, 10, 2, 9, 3}, {5, 8, 5, 11, 2, 8, 3, 11}, {4, 9, 0, 0, 7, 9, 2, 2},
+    {1, 1, 8, 5, 3, 3, 8, 6}, {11, 5, 0, 0, 11, 6, 2, 2}, {1, 1, 4, 10, 3, 3, 7, 10}};
+__constant__ unsigned char c_hil[12][8] = {
+    {0, 7, 3, 4, 1, 6, 2, 5}, {4, 3, 7, 0, 5, 2, 6, 1}, {6, 1, 5, 2, 7, 0, 4, 3},
+    {2, 5, 1, 6, 3, 4, 0, 7}, {0, 1, 7, 6, 3, 2, 4, 5}, {6, 7, 1, 0, 5, 4, 2, 3},
+    {2, 3, 5, 4, 1, 0, 6, 7}, {4, 5, 3, 2, 7, 6, 0, 1}, {0, 3, 1, 2, 7, 4, 6, 5},
+    {2, 1, 3, 0, 5, 6, 4, 7}, {4, 7, 5, 6, 3, 0, 2, 1}, {6, 5, 7, 4, 1, 2, 0, 3}};
+
+// i = (uint)(MASK * (p - o) / maxSpan) in FP64, truncated (Body.cpp:14-16).
+__device__ inline uint64_t quantise(float p, double o, double s)
+{
+    if (!(s > 0.0)) return 0;
+    double t = kMaskCoord * ((double)p - o);
+    t = t / s;
+    return (uint64_t)t;
+}
+
+// Spreads the low 21 bits of v to every third bit (bit j -> bit 3j).
+__device__ inline uint64_t spread3(uint64_t v)
+{
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x001f00000000ffffull;
+    v = (v | (v << 16)) & 0x001f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+// Body::getKey: 2^63 + sum_j 8^j (x_j + 2 y_j + 4 z_j) (Body.cpp:19-31).
+__device__ inline uint64_t morton_key(float x, float y, float z, const double *ext)
+{
+    uint64_t ix = quantise(x, ext[0], ext[3]);
+    uint64_t iy = quantise(y, ext[1], ext[3]);
+    uint64_t iz = quantise(z, ext[2], ext[3]);
+    return (1ull << 63) | spread3(ix) | (spread3(iy) << 1) | (spread3(iz) << 2);
+}
+
+__device__ inline uint64_t hilbert_of(uint64_t lebesgue)
+{
+    uint64_t h = 0;
+    int dir = 0;
+    for (int lvl = kKeyLevels; lvl > 0; --lvl) {
+        int cell = (int)((lebesgue >> ((lvl - 1) * 3)) & 7u);
+        h = (h << 3) + c_hil[dir][cell];
+        dir = c_dir[dir][cell];
+    }
+    return h;
+}
+
+__global__ void __launch_bounds__(256) keys_kernel(
+    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z, int n,
+    const DeviceState *__restrict__ st, int key_type, uint64_t *__restrict__ keys)
+{
+    const double ext[4] = {st->ext[0], st->ext[1], st->ext[2], st->ext[3]};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += blockDim.x * gridDim.x) {
+        uint64_t k = morton_key(x[i], y[i], z[i], ext);
+        if (key_type == NBX_KEY_HILBERT) k = hilbert_of(k & ~(1ull << 63));
+        keys[i] = k;
+    }
+}
+
+// updateKernel (Kernels.cu:581-600): v += dt a; x += d dt v, with explicit fused
+// multiply-adds so that the result does not depend on compiler contraction.
+__global__ void __launch_bounds__(256) update_kernel(
+    float *__restrict__ x, float *__restrict__ y, float *__restrict__ z, float *__restrict__ vx,
+    float *__restrict__ vy, float *__restrict__ vz, const float *__restrict__ ax,
+    const float *__restrict__ ay, const float *__restrict__ az, int n, float dt, float d)
+{
+    const float ddt = d * dt;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += blockDim.x * gridDim.x) {
+        float u = fmaf(dt, ax[i], vx[i]);
+        float v = fmaf(dt, ay[i], vy[i]);
+        float w = fmaf(dt, az[i], vz[i]);
+        vx[i] = u; vy[i] = v; vz[i] = w;
+        x[i] = fmaf(ddt, u, x[i]);
+        y[i] = fmaf(ddt, v, y[i]);
+        z[i] = fmaf(ddt, w, z[i]);
+    }
+}
+
+__global__ void fill_f32_kernel(float *p, float v, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)blockDim.x * gridDim.x)
+        p[i] = v;
+}
+
+static inline int grid_for(int64_t work, int block, int cap)
+{
+    int64_t g = (work + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+void launch_reset(nbx_context *c, hipStream_t s, int *mutex, float *x, float *y, float *z, float *mass,
+                  int *count, int *start, int *sorted, int *child, int *index, float *minX, float *maxX,
+                  float *minY, float *maxY, float *minZ, float *maxZ, int n, int m)
+{
+    const int block = 256;
+    reset_kernel<<<grid_for(m, block, 8192), block, 0, s>>>(mutex, x, y, z, mass, count, start, sorted, child,
+                                                            index, minX, maxX, minY, maxY, minZ, maxZ, n, m,
+                                                            c->st, c->arrive);
+}
+
+void launch_bbox(nbx_context *c, hipStream_t s, const float *x, const float *y, const float *z,
+                 float *minX, float *maxX, float *minY, float *maxY, float *minZ, float *maxZ, int n)
+{
+    const int block = 256;
+    int g = grid_for(n, block * 8, c->bbox_blocks);
+    bbox_kernel<<<g, block, 0, s>>>(x, y, z, n, c->bbox_part, c->st, minX, maxX, minY, maxY, minZ, maxZ);
+}
+
+void launch_keys(nbx_context *c, hipStream_t s, const float *x, const float *y, const float *z, int n,
+                 int key_type, uint64_t *keys)
+{
+    const int block = 256;
+    keys_kernel<<<grid_for(n, block, 65536), block, 0, s>>>(x, y, z, n, c->st, key_type, keys);
+}
+
+void launch_update(hipStream_t s, int block, float *x, float *y, float *z, float *vx, float *vy, float *vz,
+                   const float *ax, const float *ay, const float *az, int n, float dt, float d)
+{
+    if (block <= 0) block = 256;
+    update_kernel<<<grid_for(n, block, 65536), block, 0, s>>>(x, y, z, vx, vy, vz, ax, ay, az, n, dt, d);
+}
+
+void launch_fill_f32(hipStream_t s, float *p, float v, int64_t n)
+{
+    if (n <= 0) return;
+    fill_f32_kernel<<<grid_for(n, 256, 4096), 256, 0, s>>>(p, v, n);
+}
+
+}  // namespace nbx

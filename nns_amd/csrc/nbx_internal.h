@@ -1,0 +1,103 @@
+// nbx_internal.h -- context layout and kernel launchers shared by the nbx sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/nbx.h"
+
+namespace nbx {
+
+// Key geometry: 21 levels of 3 bits (Body.h:32 MASK_COORD2KEY = 2^21 - 1).
+constexpr int kKeyLevels = 21;
+constexpr double kMaskCoord = 2097151.0;
+
+// Radix sort: 8-bit digits, 256 threads x 16 items per tile.
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+
+// Force traversal: one body per lane, 64-body groups, depth-first stack per wave.
+// A cell at level L is opened at most once per path and pushes at most 8 children,
+// so the stack never holds more than 8 + 7*21 = 155 entries.
+constexpr int kForceStack = 160;
+constexpr int kRootRef = 0x7fffffff;
+
+// Device-side error latch bits (mapped to NBX_ERR_* on the host).
+constexpr int kErrCapacity = 1;
+constexpr int kErrBucket = 2;
+constexpr int kErrStack = 4;
+constexpr int kErrNonFinite = 8;
+
+struct DeviceState {
+    double ext[4];      // xMin, yMin, zMin, maxSpan
+    double root64[4];   // root mass, com x, y, z
+    float4 root4;       // root com x, y, z, mass (float)
+    int err;            // latched kErr* bits
+    unsigned bbox_ticket;
+};
+
+}  // namespace nbx
+
+struct nbx_context {
+    nbx_params p{};
+    int device = 0;
+    int n_max = 0, m_max = 0;
+    std::string last_error;
+
+    // context-owned device memory
+    nbx::DeviceState *st = nullptr;
+    float *bbox_part = nullptr;  // 6 floats per bbox block
+    int bbox_blocks = 0;
+    uint64_t *keys = nullptr, *skeys = nullptr, *kbuf = nullptr;
+    int32_t *perm = nullptr, *vbuf = nullptr, *pos = nullptr;
+    uint32_t *hist = nullptr;
+    int hist_tiles = 0;
+    int32_t *cell_level = nullptr, *parent = nullptr, *arrive = nullptr;
+    double *mass64 = nullptr, *mom64 = nullptr;
+    float4 *nodes4 = nullptr;
+    int32_t *tchild = nullptr;
+    int64_t *counts = nullptr;
+    float *acc_scratch = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t stage_ev[8] = {};
+
+    // tree bound by nbx_build_tree for the later stages
+    float *x = nullptr, *y = nullptr, *z = nullptr, *mass = nullptr;
+    int *child = nullptr, *count = nullptr, *start = nullptr, *index = nullptr;
+    int n = 0, m = 0;
+    int stage = 0;  // 0 reset, 1 bbox, 2 built, 3 summarised, 4 sorted, 5 forces
+};
+
+namespace nbx {
+
+// launchers (k_basic.hip)
+void launch_reset(nbx_context *c, hipStream_t s, int *mutex, float *x, float *y, float *z, float *mass,
+                  int *count, int *start, int *sorted, int *child, int *index, float *minX, float *maxX,
+                  float *minY, float *maxY, float *minZ, float *maxZ, int n, int m);
+void launch_bbox(nbx_context *c, hipStream_t s, const float *x, const float *y, const float *z,
+                 float *minX, float *maxX, float *minY, float *maxY, float *minZ, float *maxZ, int n);
+void launch_keys(nbx_context *c, hipStream_t s, const float *x, const float *y, const float *z, int n,
+                 int key_type, uint64_t *keys);
+void launch_update(hipStream_t s, int block, float *x, float *y, float *z, float *vx, float *vy, float *vz,
+                   const float *ax, const float *ay, const float *az, int n, float dt, float d);
+void launch_fill_f32(hipStream_t s, float *p, float v, int64_t n);
+
+// launchers (k_radix.hip)
+// Sorts keys_in (not modified) into keys_out / perm_out; writes the inverse permutation to
+// inv_out when not null.  tmp_k / tmp_v: scratch of n elements.
+void launch_radix_sort(nbx_context *c, hipStream_t s, const uint64_t *keys_in, int n, uint64_t *keys_out,
+                       int32_t *perm_out, uint64_t *tmp_k, int32_t *tmp_v, int32_t *inv_out);
+
+// launchers (k_tree.hip)
+void launch_build(nbx_context *c, hipStream_t s, int n, int m, int *child, int *index);
+void launch_summarize(nbx_context *c, hipStream_t s);
+void launch_layout(nbx_context *c, hipStream_t s, int *sorted);
+
+// launchers (k_force.hip)
+void launch_force(nbx_context *c, hipStream_t s, float *ax, float *ay, float *az, float g, int64_t *counts);
+
+}  // namespace nbx

@@ -1,0 +1,263 @@
+"""GPU parity: every stage of the HIP engine against the CPU oracle on the same
+seeded inputs, through the C ABI (nns_amd.KernelsWrapper).
+
+Bars:
+  bounding box, keys, sorted order, tree cells and child slots, cell counts and
+  start positions, FP64 cell mass / centre of mass, interaction counts,
+  update: bit-exact;
+  accelerations: FP32 against the oracle's FP64 evaluation of the same
+  interaction lists, RMS relative error <= 1e-5 and max <= 2e-3 (the max is
+  set by FP32 differences of nearby coordinates, |x| eps_32 / |d|).
+"""
+import numpy as np
+import pytest
+
+import nns_amd
+
+pytestmark = pytest.mark.gpu
+
+THETA = np.float32(0.5)
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a visible GPU"
+    return torch
+
+
+def make_case(kind, n, seed=1):
+    if kind in ("star_disk", "plummer", "binary_disk"):
+        return nns_amd.initial_conditions(kind, n, seed)
+    r = np.random.default_rng(seed)
+    if kind == "cluster":  # two tight clusters and a halo: deep trees and chains
+        a = r.normal(0, 1e-4, (n // 3, 3)) + 0.3
+        b = r.normal(0, 1e-3, (n // 3, 3)) - 0.2
+        c = r.normal(0, 1.0, (n - 2 * (n // 3), 3))
+        p = np.vstack([a, b, c]).astype(np.float32)
+    elif kind == "coincident":  # groups of 1..6 bodies at identical positions
+        base = r.normal(0, 1, (n, 3)).astype(np.float32)
+        idx = np.repeat(np.arange(n), r.integers(1, 7, n))[:n]
+        p = base[idx]
+    else:
+        raise ValueError(kind)
+    z0 = np.zeros(n, np.float32)
+    m = r.uniform(0.5, 1.5, n).astype(np.float32) / n
+    return p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), z0, z0.copy(), z0.copy(), m
+
+
+def run_to_forces(x, y, z, vx, vy, vz, m, params=None, m_nodes=None):
+    bh = nns_amd.BarnesHut(x, y, z, vx, vy, vz, m, params=params, m=m_nodes)
+    a, kw, n, mm = bh.arr, bh.kw, bh.n, bh.m
+    kw.resetArrays(a["mutex"], a["x"], a["y"], a["z"], a["mass"], a["count"], a["start"], a["sorted"],
+                   a["child"], a["index"], a["minX"], a["maxX"], a["minY"], a["maxY"], a["minZ"], a["maxZ"], n, mm)
+    kw.computeBoundingBox(a["mutex"], a["x"], a["y"], a["z"], a["minX"], a["maxX"], a["minY"], a["maxY"],
+                          a["minZ"], a["maxZ"], n)
+    kw.buildTree(a["x"], a["y"], a["z"], a["mass"], a["count"], a["start"], a["child"], a["index"], a["minX"],
+                 a["maxX"], a["minY"], a["maxY"], a["minZ"], a["maxZ"], n, mm)
+    kw.centreOfMass(a["x"], a["y"], a["z"], a["mass"], a["index"], n)
+    kw.sort(a["count"], a["start"], a["sorted"], a["child"], a["index"], n)
+    kw.computeForces(a["x"], a["y"], a["z"], a["vx"], a["vy"], a["vz"], a["ax"], a["ay"], a["az"], a["mass"],
+                     a["sorted"], a["child"], a["minX"], a["maxX"], n, bh.ctx.params.gravity)
+    bh.ctx.check()
+    return bh
+
+
+def gpu_cells(bh):
+    n = bh.n
+    index = int(bh.arr["index"].item())
+    nc = index - n
+    lvl = bh.ctx.copy_buffer(nns_amd.BUF_CELL_LEVEL, index, np.int32)[n:]
+    m64 = bh.ctx.copy_buffer(nns_amd.BUF_CELL_MASS64, index, np.float64)[n:]
+    com = bh.ctx.copy_buffer(nns_amd.BUF_CELL_COM64, 3 * index, np.float64).reshape(index, 3)[n:]
+    start = bh.host("start", index)[n:]
+    count = bh.host("count", index)[n:]
+    child = bh.host("child", 8 * index).reshape(index, 8)
+    root = bh.ctx.copy_buffer(nns_amd.BUF_ROOT, 4, np.float64)
+    return dict(nc=nc, level=lvl, mass=m64, com=com, start=start, count=count, child=child, root=root,
+                fx=bh.host("x", index)[n:], fy=bh.host("y", index)[n:], fz=bh.host("z", index)[n:],
+                fm=bh.host("mass", index)[n:])
+
+
+def check_tree(bh, t):
+    """Cells, children, FP64 summaries and counts against the oracle's key tree."""
+    n = bh.n
+    g = gpu_cells(bh)
+    # oracle cell 0 is the root; GPU cells are [n, index)
+    assert g["nc"] == t.ncell - 1
+    okey = {(int(t.level[c]), int(t.start[c])): c for c in range(1, t.ncell)}
+    gid = {}
+    for k in range(g["nc"]):
+        key = (int(g["level"][k]), int(g["start"][k]))
+        assert key in okey, key
+        gid[n + k] = okey[key]
+    oc = np.array([gid[n + k] for k in range(g["nc"])], np.int64)
+    assert np.array_equal(g["count"], t.count[oc])
+    assert np.array_equal(g["mass"], t.mass[oc])          # FP64, bit for bit
+    assert np.array_equal(g["com"], t.com[oc])
+    assert np.array_equal(g["fx"], t.c32[oc, 0]) and np.array_equal(g["fm"], t.c32[oc, 3])
+    assert g["root"][0] == t.mass[0] and np.array_equal(g["root"][1:], t.com[0])
+
+    def enc(v):  # GPU child value -> oracle encoding
+        return v if v < n else -2 - gid[v]
+    for s in range(8):
+        v = int(g["child"][0, s])
+        assert (v if v < 0 else enc(v)) == t.child[0, s]
+    ch = g["child"][n:]
+    for k in range(g["nc"]):
+        got = [v if v < 0 else enc(int(v)) for v in ch[k]]
+        assert got == list(t.child[oc[k]]), (k, got, t.child[oc[k]])
+
+
+def check_forces(bh, t, oracle, eps2, tol_rms=1e-5, tol_max=2e-3):
+    a_orc, cnt_orc = t.accel(bh.ctx.params.theta, eps2, mac_mode=1)
+    ag = np.stack([bh.host("ax"), bh.host("ay"), bh.host("az")], 1).astype(np.float64)
+    G = bh.ctx.params.gravity
+    norm = np.linalg.norm(a_orc, axis=1)
+    ok = norm > 0
+    rel = np.linalg.norm(ag[ok] - G * a_orc[ok], axis=1) / (G * norm[ok])
+    assert np.sqrt((rel ** 2).mean()) <= tol_rms, np.sqrt((rel ** 2).mean())
+    assert rel.max() <= tol_max, rel.max()
+    cnt_gpu = bh.count_interactions().cpu().numpy()
+    assert np.array_equal(cnt_gpu, cnt_orc)
+    return rel, cnt_orc
+
+
+CASES = [("star_disk", 4096), ("star_disk", 32768), ("plummer", 20000), ("binary_disk", 50000),
+         ("cluster", 9000), ("coincident", 5000), ("plummer", 2), ("plummer", 3), ("plummer", 130)]
+
+
+@pytest.mark.parametrize("kind,n", CASES)
+def test_pipeline_matches_oracle(oracle, kind, n):
+    _torch()
+    print(f"case {kind} n={n}", flush=True)
+    x, y, z, vx, vy, vz, m = make_case(kind, n)
+    params = nns_amd.default_params(theta=float(THETA))
+    bh = run_to_forces(x, y, z, vx, vy, vz, m, params)
+    # bounding box and extent
+    mm, ext = oracle.bbox(x, y, z)
+    got = np.array([bh.arr[k].item() for k in ("minX", "maxX", "minY", "maxY", "minZ", "maxZ")], np.float32)
+    assert np.array_equal(got, mm)
+    assert np.array_equal(bh.ctx.copy_buffer(nns_amd.BUF_EXTENT, 4, np.float64), ext)
+    # keys and sorted order
+    t = oracle.KeyTree(x, y, z, m)
+    assert t.err == 0
+    keys = bh.ctx.copy_buffer(nns_amd.BUF_KEYS, n, np.uint64)
+    assert np.array_equal(keys, oracle.keys_morton(x, y, z))
+    assert np.array_equal(bh.ctx.copy_buffer(nns_amd.BUF_SORTED_KEYS, n, np.uint64), t.skeys)
+    assert np.array_equal(bh.host("sorted"), t.perm)
+    check_tree(bh, t)
+    check_forces(bh, t, oracle, float(np.float32(np.float32(params.softening) ** 2)))
+
+
+def test_unsoftened_and_no_kahan(oracle):
+    _torch()
+    x, y, z, vx, vy, vz, m = make_case("coincident", 3000, seed=5)
+    params = nns_amd.default_params(theta=0.6, softening=0.0, kahan=0)
+    bh = run_to_forces(x, y, z, vx, vy, vz, m, params)
+    t = oracle.KeyTree(x, y, z, m)
+    check_tree(bh, t)
+    check_forces(bh, t, oracle, 0.0)
+
+
+def test_keys_hilbert_and_sort_extension(oracle):
+    torch = _torch()
+    x, y, z, *_ = make_case("plummer", 70000, seed=2)
+    ctx = nns_amd.Context(70000)
+    kw = nns_amd.KernelsWrapper(ctx)
+    dev = torch.device("cuda")
+    tx, ty, tz = (torch.as_tensor(a, device=dev) for a in (x, y, z))
+    mins = [torch.zeros(1, device=dev) for _ in range(6)]
+    mutex = torch.zeros(1, dtype=torch.int32, device=dev)
+    kw.computeBoundingBox(mutex, tx, ty, tz, *mins, 70000)
+    keys = torch.zeros(70000, dtype=torch.int64, device=dev)
+    kw.computeKeys(tx, ty, tz, 70000, keys, nns_amd.KEY_HILBERT)
+    hk = keys.cpu().numpy().view(np.uint64)
+    assert np.array_equal(hk, oracle.keys_hilbert(oracle.keys_morton(x, y, z)))
+    sk = torch.zeros_like(keys)
+    perm = torch.zeros(70000, dtype=torch.int32, device=dev)
+    kw.sortKeys(keys, 70000, sk, perm)
+    p_orc, sk_orc = oracle.sort_stable(hk)
+    assert np.array_equal(perm.cpu().numpy(), p_orc)
+    assert np.array_equal(sk.cpu().numpy().view(np.uint64), sk_orc)
+
+
+def test_update_bit_exact_and_step_matches_stages(oracle):
+    torch = _torch()
+    x, y, z, vx, vy, vz, m = make_case("star_disk", 20000, seed=3)
+    p = nns_amd.default_params()
+    bh = run_to_forces(x, y, z, vx, vy, vz, m, p)
+    ax, ay, az = bh.host("ax"), bh.host("ay"), bh.host("az")
+    a = bh.arr
+    bh.kw.update(a["x"], a["y"], a["z"], a["vx"], a["vy"], a["vz"], a["ax"], a["ay"], a["az"], bh.n,
+                 p.timestep, p.dampening)
+    torch.cuda.synchronize()
+    ex = oracle.update(x, y, z, vx, vy, vz, ax, ay, az, p.timestep, p.dampening)
+    for k, e in zip(("x", "y", "z", "vx", "vy", "vz"), ex):
+        assert np.array_equal(bh.host(k), e), k
+    # nbx_step on a fresh copy gives the same state as the seven stage calls
+    bh2 = nns_amd.BarnesHut(x, y, z, vx, vy, vz, m, params=p)
+    ms = bh2.update(stage_times=True)
+    assert len(ms) == 7 and all(v >= 0 for v in ms)
+    for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"):
+        assert np.array_equal(bh2.host(k), bh.host(k)), k
+
+
+def test_ten_step_run_tracks_oracle(oracle):
+    """Plain-method CPU case shape (star + disk, 10 steps) at reduced N: the GPU
+    trajectory and the oracle's FP64-force trajectory stay within 1e-5."""
+    _torch()
+    n = 8192
+    x, y, z, vx, vy, vz, m = make_case("star_disk", n, seed=4)
+    p = nns_amd.default_params()
+    bh = nns_amd.BarnesHut(x, y, z, vx, vy, vz, m, params=p)
+    st = [a.copy() for a in (x, y, z, vx, vy, vz)]
+    eps2 = float(np.float32(np.float32(p.softening) ** 2))
+    for _ in range(10):
+        bh.update()
+        t = oracle.KeyTree(st[0], st[1], st[2], m)
+        acc, _ = t.accel(0.5, eps2, mac_mode=1)
+        st = oracle.update(*st, acc[:, 0].astype(np.float32), acc[:, 1].astype(np.float32),
+                           acc[:, 2].astype(np.float32), p.timestep, p.dampening)
+    bh.ctx.check()
+    pos = np.stack([bh.host("x"), bh.host("y"), bh.host("z")], 1)
+    ref = np.stack(st[:3], 1)
+    assert np.abs(pos - ref).max() < 1e-5
+
+
+def test_error_paths_do_not_fault(oracle):
+    torch = _torch()
+    # capacity: m barely above n
+    x, y, z, vx, vy, vz, m = make_case("plummer", 5000, seed=6)
+    bh = nns_amd.BarnesHut(x, y, z, vx, vy, vz, m, m=5000 + 50)
+    with pytest.raises(nns_amd.NbxError) as e:
+        bh.update()
+        bh.ctx.check()
+    assert e.value.code == -2
+    # bucket overflow: nine bodies on one point
+    x, y, z, vx, vy, vz, m = make_case("plummer", 1000, seed=6)
+    x[:9], y[:9], z[:9] = x[0], y[0], z[0]
+    bh = nns_amd.BarnesHut(x, y, z, vx, vy, vz, m)
+    with pytest.raises(nns_amd.NbxError) as e:
+        bh.update()
+        bh.ctx.check()
+    assert e.value.code == -3
+    # non-finite input
+    x, y, z, vx, vy, vz, m = make_case("plummer", 1000, seed=6)
+    x[17] = np.nan
+    bh = nns_amd.BarnesHut(x, y, z, vx, vy, vz, m)
+    with pytest.raises(nns_amd.NbxError) as e:
+        bh.update()
+        bh.ctx.check()
+    assert e.value.code == -5
+    # stage order
+    x, y, z, vx, vy, vz, m = make_case("plummer", 1000, seed=6)
+    bh = nns_amd.BarnesHut(x, y, z, vx, vy, vz, m)
+    a = bh.arr
+    with pytest.raises(nns_amd.NbxError) as e:
+        bh.kw.computeForces(a["x"], a["y"], a["z"], a["vx"], a["vy"], a["vz"], a["ax"], a["ay"], a["az"],
+                            a["mass"], a["sorted"], a["child"], a["minX"], a["maxX"], bh.n, 1.0)
+    assert e.value.code == -7
+    # the context recovers after a reset
+    bh.update()
+    bh.ctx.check()
+    torch.cuda.synchronize()

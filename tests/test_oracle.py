@@ -1,0 +1,272 @@
+"""Pins the CPU oracle (oracle/nbx_oracle.c) before it is trusted as the GPU checker.
+
+Every function of the oracle is checked against at least one of: the golden
+vectors written by the reference's own compiled code (tests/golden/), a
+closed form, an invariant, or brute force on a tiny input.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+KEY_CASES = ["cloud", "disk", "dups", "lattice"]
+TREE_CASES = ["blob_t05", "blob_t07_soft", "disk_t05"]
+
+
+# ------------------------------------------------------------------ keys
+@pytest.mark.parametrize("case", KEY_CASES)
+def test_keys_match_reference(oracle, case):
+    g = golden(f"keys_{case}.npz")
+    k = oracle.keys_morton(g["x"], g["y"], g["z"])
+    assert np.array_equal(k, g["keys"])
+
+
+@pytest.mark.parametrize("case", KEY_CASES)
+def test_stable_order_matches_reference_map(oracle, case):
+    """std::map<key, Body*> keeps one body per key, the last one inserted
+    (Interaction.cpp:166); the stable order restricted to the last body of each
+    key must equal the reference's map order."""
+    g = golden(f"keys_{case}.npz")
+    perm, sk = oracle.sort_stable(g["keys"])
+    assert np.all(sk[1:] >= sk[:-1])
+    last = np.ones(len(sk), bool)
+    last[:-1] = sk[1:] != sk[:-1]
+    assert np.array_equal(perm[last], g["map_order"])
+
+
+def test_key_corners_and_axes(oracle):
+    ext = np.array([0.0, 0.0, 0.0, 1.0])
+    k = lambda x, y, z: int(oracle.lib().orc_morton_key(x, y, z, ext))
+    assert k(0, 0, 0) == 1 << 63
+    assert k(1, 1, 1) == (1 << 64) - 1
+    # one axis at the maximum: exactly the bits of that axis (x -> 3j, y -> 3j+1, z -> 3j+2)
+    xs = sum(1 << (3 * j) for j in range(21))
+    assert k(1, 0, 0) == (1 << 63) | xs
+    assert k(0, 1, 0) == (1 << 63) | (xs << 1)
+    assert k(0, 0, 1) == (1 << 63) | (xs << 2)
+    # upper half of the root on x sets digit 0 (bits 60..62) to 1
+    assert (k(0.75, 0.1, 0.1) >> 60) & 7 == 1
+    assert (k(0.1, 0.75, 0.1) >> 60) & 7 == 2
+    assert (k(0.1, 0.1, 0.75) >> 60) & 7 == 4
+
+
+def test_keys_monotone_along_axis(oracle):
+    x = np.linspace(0, 1, 1001).astype(np.float32)
+    z0 = np.zeros_like(x)
+    k = oracle.keys_morton(x, z0, z0)
+    assert np.all(np.diff(k.astype(object)) >= 0)
+
+
+def _lebesgue(ix, iy, iz, L):
+    key = 0
+    for j in range(L):
+        key |= (((ix >> j) & 1) | (((iy >> j) & 1) << 1) | (((iz >> j) & 1) << 2)) << (3 * j)
+    return key << (3 * (21 - L))
+
+
+@pytest.mark.parametrize("L", [1, 2, 3])
+def test_hilbert_is_a_face_adjacent_bijection(oracle, L):
+    """Lebesgue2Hilbert (SFC/src/Tree.cpp:82-104) restricted to level L must be
+    a bijection onto 0..8^L-1 whose consecutive indices are face neighbours."""
+    side = 1 << L
+    cells = {}
+    for ix in range(side):
+        for iy in range(side):
+            for iz in range(side):
+                h = oracle.lebesgue2hilbert(_lebesgue(ix, iy, iz, L), L) >> (3 * (21 - L))
+                cells[h] = (ix, iy, iz)
+    assert sorted(cells) == list(range(side ** 3))
+    for h in range(side ** 3 - 1):
+        a, b = np.array(cells[h]), np.array(cells[h + 1])
+        assert np.abs(a - b).sum() == 1, (h, cells[h], cells[h + 1])
+
+
+def test_hilbert_full_keys_order_consistent(oracle):
+    """Hilbert keys of full-depth Morton keys are distinct for distinct cells and
+    keep whole octants contiguous (a level-1 prefix maps to a level-1 prefix)."""
+    r = np.random.default_rng(3)
+    x, y, z = (r.uniform(0, 1, 4000).astype(np.float32) for _ in range(3))
+    mk = oracle.keys_morton(x, y, z)
+    hk = oracle.keys_hilbert(mk)
+    assert len(np.unique(hk)) == len(np.unique(mk))
+    top_m = (mk >> np.uint64(60)) & np.uint64(7)
+    top_h = hk >> np.uint64(60)
+    for d in range(8):
+        assert len(np.unique(top_h[top_m == d])) <= 1
+
+
+def test_sort_stable_equals_numpy(oracle):
+    r = np.random.default_rng(5)
+    keys = (r.integers(0, 50, 20000).astype(np.uint64) << np.uint64(40)) | np.uint64(1 << 63)
+    perm, sk = oracle.sort_stable(keys)
+    ref = np.argsort(keys, kind="stable")
+    assert np.array_equal(perm, ref.astype(np.int32))
+    assert np.array_equal(sk, keys[ref])
+
+
+# ------------------------------------------------ reference tree restatement
+@pytest.mark.parametrize("case", TREE_CASES)
+def test_geo_tree_matches_reference_bit_for_bit(oracle, case):
+    g = golden(f"tree_{case}.npz")
+    t = oracle.GeoTree(g["x"], g["y"], g["z"], g["m"], g["center"], float(g["length"]))
+    assert t.err == 0
+    dv = t.accel_reference_si(float(g["theta"]), float(g["softening_m"]))
+    assert np.array_equal(dv, g["dv"])
+    # node sets: level, geometry, mass and centre of mass, exactly
+    ours = np.concatenate([t.ints[:, :1].astype(np.float64), t.dbl], axis=1)
+    ref = np.concatenate([g["node_level"][:, None].astype(np.float64), g["node_d"]], axis=1)
+    o1 = np.lexsort(ours[:, 4::-1].T)
+    o2 = np.lexsort(ref[:, 4::-1].T)
+    assert np.array_equal(ours[o1], ref[o2])
+
+
+def _key_cells_geometry(t):
+    """Level and geometric centre of every key-tree cell from its key prefix."""
+    lvl = t.level
+    key = t.skeys[t.start]
+    l = t.lcell0 / (2.0 ** lvl)
+    out = np.zeros((t.ncell, 3))
+    for c in range(t.ncell):
+        L = int(lvl[c])
+        pre = int(key[c]) >> (63 - 3 * L) if L else 0
+        ix = iy = iz = 0
+        for j in range(L):
+            d = (pre >> (3 * j)) & 7
+            ix |= (d & 1) << j; iy |= ((d >> 1) & 1) << j; iz |= ((d >> 2) & 1) << j
+        out[c] = t.ext[:3] + (np.array([ix, iy, iz]) + 0.5) * l[c]
+    return out
+
+
+@pytest.mark.parametrize("case", TREE_CASES)
+def test_key_tree_is_the_reference_tree(oracle, case):
+    """With the root octant placed on the key grid's root cell, the reference's
+    midpoint octree and the key-prefix tree have the same cells, masses and
+    centres of mass, and the same Barnes-Hut accelerations."""
+    g = golden(f"tree_{case}.npz")
+    kt = oracle.KeyTree(g["x"], g["y"], g["z"], g["m"])
+    gt = oracle.GeoTree(g["x"], g["y"], g["z"], g["m"], g["center"], float(g["length"]))
+    assert kt.err == 0 and gt.err == 0
+    internal = gt.ints[:, 1] == 0
+    assert internal.sum() == kt.ncell
+    geo = np.concatenate([gt.ints[internal, :1], gt.dbl[internal][:, [0, 1, 2, 4, 5, 6, 7]]], axis=1)
+    cen = _key_cells_geometry(kt)
+    key = np.concatenate([kt.level[:, None], cen, kt.mass[:, None], kt.com], axis=1)
+    o1 = np.lexsort(np.round(geo[:, 3::-1].T, 9))
+    o2 = np.lexsort(np.round(key[:, 3::-1].T, 9))
+    assert np.array_equal(geo[o1, 0], key[o2, 0])
+    np.testing.assert_allclose(key[o2, 1:4], geo[o1, 1:4], rtol=0, atol=1e-12 * float(g["length"]))
+    np.testing.assert_allclose(key[o2, 4], geo[o1, 4], rtol=1e-13)
+    np.testing.assert_allclose(key[o2, 5:], geo[o1, 5:], rtol=0, atol=1e-13 * float(g["length"]))
+    eps2 = float(g["softening_au"]) ** 2
+    a_key, _ = kt.accel(float(g["theta"]), eps2, mac_mode=0)
+    a_geo = gt.accel_plummer(float(g["theta"]), eps2)
+    rel = np.linalg.norm(a_key - a_geo, axis=1) / np.linalg.norm(a_geo, axis=1)
+    assert rel.max() < 1e-11
+
+
+@pytest.mark.parametrize("case", TREE_CASES)
+def test_reference_si_units_equal_plummer_g1(oracle, case):
+    """With zero softening the reference's SI increment is TIME_STEP*G/TO_METERS^2
+    times the G = 1 acceleration of the same tree walk (Interaction.cpp:17-55)."""
+    g = golden(f"tree_{case}.npz")
+    if float(g["softening_au"]) != 0.0:
+        pytest.skip("softened reference kernel is d(d^2+eps^2), not Plummer")
+    gt = oracle.GeoTree(g["x"], g["y"], g["z"], g["m"], g["center"], float(g["length"]))
+    a = gt.accel_plummer(float(g["theta"]), 0.0)
+    scale = oracle.G_SI / oracle.TO_METERS ** 2
+    np.testing.assert_allclose(g["dv"], a * scale, rtol=1e-12, atol=0)
+
+
+def test_key_tree_invariants_brute_force(oracle):
+    r = np.random.default_rng(7)
+    n = 700
+    x, y, z = (r.normal(0, 1, n).astype(np.float32) for _ in range(3))
+    m = r.uniform(0.1, 1.0, n).astype(np.float32)
+    t = oracle.KeyTree(x, y, z, m)
+    assert t.err == 0
+    assert t.count[0] == n and t.start[0] == 0 and t.level[0] == 0
+    np.testing.assert_allclose(t.mass[0], m.astype(np.float64).sum(), rtol=1e-14)
+    com = np.array([(m.astype(np.float64) * a.astype(np.float64)).sum() for a in (x, y, z)]) / t.mass[0]
+    np.testing.assert_allclose(t.com[0], com, rtol=1e-13, atol=1e-15)
+    keys = oracle.keys_morton(x, y, z)
+    for c in range(t.ncell):
+        L = int(t.level[c])
+        members = t.perm[t.start[c]:t.start[c] + t.count[c]]
+        pre = keys[members] >> np.uint64(63 - 3 * L) if L else np.zeros(len(members), np.uint64)
+        assert len(np.unique(pre)) == 1                      # one prefix per cell
+        full = keys >> np.uint64(63 - 3 * L) if L else np.zeros(n, np.uint64)
+        assert (full == pre[0]).sum() == t.count[c]           # and every body with it
+        mb = m[members].astype(np.float64)
+        np.testing.assert_allclose(t.mass[c], mb.sum(), rtol=1e-14)
+        for ch in t.child[c]:
+            if ch <= -2:
+                assert t.level[-2 - ch] == L + 1 and t.parent[-2 - ch] == c
+
+
+def test_theta_zero_is_direct_sum(oracle):
+    r = np.random.default_rng(8)
+    n = 600
+    x, y, z = (r.normal(0, 1, n).astype(np.float32) for _ in range(3))
+    m = r.uniform(0.1, 1.0, n).astype(np.float32)
+    t = oracle.KeyTree(x, y, z, m)
+    a, cnt = t.accel(1e-12, 1e-4, mac_mode=0)
+    d = oracle.direct_accel(x, y, z, m, 1e-4)
+    np.testing.assert_allclose(a, d, rtol=1e-12, atol=1e-12 * np.abs(d).max())
+    assert np.all(cnt == n - 1)
+
+
+def test_device_predicate_mode_agrees(oracle):
+    """The FP32 opening predicate of the GPU changes the interaction list only at
+    ties; accelerations of the two modes agree to FP32 resolution."""
+    r = np.random.default_rng(9)
+    n = 3000
+    x, y, z = (r.normal(0, 1, n).astype(np.float32) for _ in range(3))
+    m = np.full(n, 1.0 / n, np.float32)
+    t = oracle.KeyTree(x, y, z, m)
+    a0, c0 = t.accel(0.5, 1e-6, mac_mode=0)
+    a1, c1 = t.accel(0.5, 1e-6, mac_mode=1)
+    rel = np.linalg.norm(a1 - a0, axis=1) / np.linalg.norm(a0, axis=1)
+    assert rel.max() < 1e-5
+    assert (c0 != c1).mean() < 1e-2
+
+
+# ------------------------------------------------------------ closed forms
+def test_direct_two_body_and_cube(oracle):
+    eps = 0.1
+    x = np.array([0.0, 2.0], np.float32); y = np.zeros(2, np.float32); z = np.zeros(2, np.float32)
+    m = np.array([1.0, 3.0], np.float32)
+    a = oracle.direct_accel(x, y, z, m, eps * eps)
+    assert a[0, 0] == pytest.approx(3.0 * 2.0 / (4.0 + eps * eps) ** 1.5, rel=1e-14)
+    assert a[1, 0] == pytest.approx(-1.0 * 2.0 / (4.0 + eps * eps) ** 1.5, rel=1e-14)
+    # body at the centre of a cube of equal masses feels no force
+    c = np.array([[i, j, k] for i in (-1, 1) for j in (-1, 1) for k in (-1, 1)] + [[0, 0, 0]], np.float32)
+    mm = np.ones(9, np.float32)
+    a = oracle.direct_accel(c[:, 0], c[:, 1], c[:, 2], mm, 0.0)
+    assert np.abs(a[8]).max() < 1e-15
+
+
+def test_bh_two_cluster_monopole(oracle):
+    """A tight cluster far away is accepted as one monopole: the force is the
+    cluster's total mass at its centre of mass."""
+    r = np.random.default_rng(10)
+    cl = r.normal(0, 1e-3, (50, 3)) + np.array([100.0, 0, 0])
+    pts = np.vstack([[[0.0, 0.0, 0.0]], cl]).astype(np.float32)
+    m = np.concatenate([[1.0], r.uniform(0.5, 1.5, 50)]).astype(np.float32)
+    t = oracle.KeyTree(pts[:, 0], pts[:, 1], pts[:, 2], m)
+    a, cnt = t.accel(0.5, 0.0, mac_mode=0, targets=[0])
+    mc = m[1:].astype(np.float64)
+    com = (mc[:, None] * pts[1:].astype(np.float64)).sum(0) / mc.sum()
+    expect = mc.sum() * com / np.linalg.norm(com) ** 3
+    np.testing.assert_allclose(a[0], expect, rtol=1e-12)
+    assert cnt[0] == 1
+
+
+def test_update_is_symplectic_euler(oracle):
+    r = np.random.default_rng(4)
+    v = [r.normal(0, 1, 100).astype(np.float32) for _ in range(9)]
+    x, y, z, vx, vy, vz = oracle.update(*v, 1e-3, 0.5)
+    for pos, vel, v0, p0, a in ((x, vx, v[3], v[0], v[6]), (y, vy, v[4], v[1], v[7]), (z, vz, v[5], v[2], v[8])):
+        ev = v0.astype(np.float64) + 1e-3 * a.astype(np.float64)
+        np.testing.assert_allclose(vel, ev, rtol=2e-7, atol=1e-9)
+        ex = p0.astype(np.float64) + np.float32(0.5 * np.float32(1e-3)) * vel.astype(np.float64)
+        np.testing.assert_allclose(pos, ex, rtol=2e-7, atol=1e-9)
