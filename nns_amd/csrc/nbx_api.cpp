@@ -204,6 +204,7 @@ int nbx_reset_arrays(nbx_context *c, int *mutex, float *x, float *y, float *z, f
     nbx::launch_reset(c, s, mutex, x, y, z, mass, count, start, sorted, child, index, minX, maxX, minY, maxY,
                       minZ, maxZ, n, m);
     c->stage = 0;
+    c->fresh = true;
     return t.finish();
 }
 
@@ -235,6 +236,8 @@ int nbx_build_tree(nbx_context *c, float *x, float *y, float *z, float *mass, in
     if (!x || !y || !z || !mass || !count || !start || !child || !index)
         return fail(c, NBX_ERR_INVALID, "null array");
     if (c->stage < 1) return fail(c, NBX_ERR_ORDER, "nbx_build_tree needs nbx_compute_bounding_box first");
+    if (!c->fresh) return fail(c, NBX_ERR_ORDER, "nbx_build_tree needs nbx_reset_arrays first");
+    c->fresh = false;
     hipSetDevice(c->device);
     hipStream_t s = (hipStream_t)stream;
     c->x = x; c->y = y; c->z = z; c->mass = mass;

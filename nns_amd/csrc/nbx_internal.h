@@ -70,6 +70,7 @@ struct nbx_context {
     int *child = nullptr, *count = nullptr, *start = nullptr, *index = nullptr;
     int n = 0, m = 0;
     int stage = 0;  // 0 reset, 1 bbox, 2 built, 3 summarised, 4 sorted, 5 forces
+    bool fresh = false;  // arrays reset since the last build (root slots are -1)
 };
 
 namespace nbx {
