@@ -186,6 +186,40 @@ int nbx_step(nbx_context *ctx, const nbx_arrays *a, void *stream, float *stage_m
 #define NBX_BUF_EXTENT 8      /* double [4] xMin, yMin, zMin, maxSpan            */
 int nbx_copy_buffer(nbx_context *ctx, int which, void *host_dst, int64_t count, void *stream);
 
+/* Times the force kernel at 2, 4 and 8 waves per block on the current tree (after nbx_sort)
+ * and keeps the fastest for this context's later nbx_compute_forces calls.  *waves_out
+ * (nullable) receives the choice.  Synchronises the stream. */
+int nbx_autotune(nbx_context *ctx, void *stream, int *waves_out);
+
+/* ---- batched ensembles (no counterpart in the reference) --------------------------- */
+
+/* nsys independent systems in one set of concatenated arrays: system s owns bodies
+ * [off_s, off_s + sys_n[s]) with off_s = sys_n[0] + ... + sys_n[s-1].  sys_n and sys_theta
+ * (per-system opening angle; NULL = p->theta) are host arrays.  Every stage is one launch
+ * for the whole ensemble; the tree of system s hangs off its own root cell. */
+typedef struct nbx_batch nbx_batch;
+int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const float *sys_theta, int device,
+                     nbx_batch **out);
+int nbx_batch_destroy(nbx_batch *b);
+const char *nbx_batch_last_error(const nbx_batch *b);
+/* Binds the caller's concatenated device arrays (N = sum of sys_n elements each). */
+int nbx_batch_bind(nbx_batch *b, float *x, float *y, float *z, float *vx, float *vy, float *vz, float *ax,
+                   float *ay, float *az, float *mass);
+/* One step of every system (reset, bounding boxes, build, summary, layout, forces, update);
+ * stage_ms (7 floats, nullable) as in nbx_step. */
+int nbx_batch_step(nbx_batch *b, void *stream, float *stage_ms);
+/* Same step, captured once into a hipGraph on `stream` (not NULL) and replayed. */
+int nbx_batch_graph_step(nbx_batch *b, void *stream);
+int nbx_batch_check(nbx_batch *b, void *stream);
+/* Per-body interaction counts of the last tree, device array of N int64. */
+int nbx_batch_count_interactions(nbx_batch *b, int64_t *counts, void *stream);
+/* nbx_autotune for the ensemble's force kernel (invalidates a captured graph). */
+int nbx_batch_autotune(nbx_batch *b, void *stream, int *waves_out);
+/* Host copy of a batch buffer: NBX_BUF_KEYS, NBX_BUF_SORTED_KEYS, NBX_BUF_PERM (N each),
+ * NBX_BUF_EXTENT (4 per system), NBX_BUF_CELL_MASS64 / NBX_BUF_CELL_COM64 / NBX_BUF_CELL_LEVEL
+ * (node index space, roots at N + s). */
+int nbx_batch_copy_buffer(nbx_batch *b, int which, void *host_dst, int64_t count, void *stream);
+
 /* Seeded initial conditions written to host arrays of length n (positions, velocities,
  * masses; the rest untouched).  The generator is xoshiro256** seeded by splitmix64. */
 int nbx_ic_star_disk(int n, uint64_t seed, float *x, float *y, float *z, float *vx, float *vy,

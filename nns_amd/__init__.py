@@ -38,7 +38,9 @@ EXPORTS = [
     "nbx_last_error", "nbx_check", "nbx_reset_arrays", "nbx_compute_bounding_box", "nbx_build_tree",
     "nbx_centre_of_mass", "nbx_sort", "nbx_compute_forces", "nbx_update", "nbx_compute_keys",
     "nbx_sort_keys", "nbx_count_interactions", "nbx_step", "nbx_copy_buffer", "nbx_ic_star_disk",
-    "nbx_ic_plummer", "nbx_ic_binary_disk",
+    "nbx_ic_plummer", "nbx_ic_binary_disk", "nbx_autotune", "nbx_batch_create", "nbx_batch_destroy",
+    "nbx_batch_last_error", "nbx_batch_bind", "nbx_batch_step", "nbx_batch_graph_step", "nbx_batch_check",
+    "nbx_batch_count_interactions", "nbx_batch_autotune", "nbx_batch_copy_buffer",
 ]
 
 
@@ -95,6 +97,19 @@ def lib():
     L.nbx_copy_buffer.argtypes = [vp, i, vp, C.c_int64, vp]
     for name in ("nbx_ic_star_disk", "nbx_ic_plummer", "nbx_ic_binary_disk"):
         getattr(L, name).argtypes = [i, C.c_uint64] + [vp] * 7
+    ip = C.POINTER(C.c_int)
+    L.nbx_autotune.argtypes = [vp, vp, ip]
+    L.nbx_batch_create.argtypes = [C.POINTER(Params), i, vp, vp, i, C.POINTER(vp)]
+    L.nbx_batch_destroy.argtypes = [vp]
+    L.nbx_batch_last_error.restype = C.c_char_p
+    L.nbx_batch_last_error.argtypes = [vp]
+    L.nbx_batch_bind.argtypes = [vp] + [vp] * 10
+    L.nbx_batch_step.argtypes = [vp, vp, fp]
+    L.nbx_batch_graph_step.argtypes = [vp, vp]
+    L.nbx_batch_check.argtypes = [vp, vp]
+    L.nbx_batch_count_interactions.argtypes = [vp, vp, vp]
+    L.nbx_batch_autotune.argtypes = [vp, vp, ip]
+    L.nbx_batch_copy_buffer.argtypes = [vp, i, vp, C.c_int64, vp]
     _lib = L
     return L
 
@@ -310,3 +325,84 @@ class BarnesHut:
     def host(self, k, count=None):
         t = self.arr[k]
         return t[: (count if count is not None else self.n)].cpu().numpy()
+
+
+class Batch:
+    """Ensemble of independent systems in concatenated device arrays
+    (nbx_batch_*).  sys_n: bodies per system; sys_theta: per-system theta."""
+
+    def __init__(self, x, y, z, vx, vy, vz, mass, sys_n, sys_theta=None, params: Params | None = None, device=0):
+        import torch
+        self.sys_n = np.ascontiguousarray(sys_n, dtype=np.int32)
+        self.S = len(self.sys_n)
+        self.N = int(self.sys_n.sum())
+        self.off = np.concatenate([[0], np.cumsum(self.sys_n)]).astype(np.int64)
+        self.params = params if params is not None else default_params()
+        th = None if sys_theta is None else np.ascontiguousarray(sys_theta, dtype=np.float32)
+        self.sys_theta = th
+        h = C.c_void_p()
+        rc = lib().nbx_batch_create(C.byref(self.params), self.S, self.sys_n.ctypes.data_as(C.c_void_p),
+                                    None if th is None else th.ctypes.data_as(C.c_void_p), int(device), C.byref(h))
+        if rc != NBX_OK:
+            raise NbxError(rc, lib().nbx_batch_last_error(None).decode())
+        self.h = h
+        self.dev = torch.device("cuda", device)
+        self.arr = {}
+        for k, v in (("x", x), ("y", y), ("z", z), ("vx", vx), ("vy", vy), ("vz", vz), ("mass", mass)):
+            self.arr[k] = torch.as_tensor(np.asarray(v, np.float32), device=self.dev).contiguous()
+        for k in ("ax", "ay", "az"):
+            self.arr[k] = torch.zeros(self.N, dtype=torch.float32, device=self.dev)
+        a = self.arr
+        self._rc(lib().nbx_batch_bind(self.h, *[a[k].data_ptr() for k in
+                                                ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "mass")]))
+
+    def _rc(self, rc):
+        if rc != NBX_OK:
+            raise NbxError(rc, lib().nbx_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nbx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, stage_times=False, stream=None):
+        ms = (C.c_float * 7)() if stage_times else None
+        self._rc(lib().nbx_batch_step(self.h, _stream(stream), ms))
+        return list(ms) if stage_times else None
+
+    def graph_step(self, stream=None):
+        self._rc(lib().nbx_batch_graph_step(self.h, _stream(stream)))
+
+    def check(self, stream=None):
+        self._rc(lib().nbx_batch_check(self.h, _stream(stream)))
+
+    def autotune(self, stream=None):
+        w = C.c_int(0)
+        self._rc(lib().nbx_batch_autotune(self.h, _stream(stream), C.byref(w)))
+        return w.value
+
+    def count_interactions(self):
+        import torch
+        out = torch.zeros(self.N, dtype=torch.int64, device=self.dev)
+        self._rc(lib().nbx_batch_count_interactions(self.h, out.data_ptr(), _stream(None)))
+        return out
+
+    def copy_buffer(self, which, count, dtype):
+        out = np.zeros(count, dtype)
+        self._rc(lib().nbx_batch_copy_buffer(self.h, which, out.ctypes.data_as(C.c_void_p), count, _stream(None)))
+        return out
+
+    def host(self, k):
+        return self.arr[k].cpu().numpy()
+
+
+def autotune(ctx: Context, stream=None) -> int:
+    w = C.c_int(0)
+    ctx._rc(lib().nbx_autotune(ctx.h, _stream(stream), C.byref(w)))
+    return w.value

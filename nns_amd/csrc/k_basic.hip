@@ -1,7 +1,7 @@
 // k_basic.hip -- streaming stages: reset, bounding box, keys, update.
 // All of them are HBM-bound: coalesced grid-stride loops, no LDS except the
 // bounding-box block reduction.
-#include "nbx_internal.h"
+#include "nbx_device.h"
 
 namespace nbx {
 
@@ -132,61 +132,6 @@ __global__ void __launch_bounds__(256) bbox_kernel(
     }
 }
 
-// Lebesgue2Hilbert state tables (MolecularDynamics/BarnesHutParallel/
-// SpaceFillingCurves/src/Tree.cpp:71-80); the values define the key.
-__constant__ unsigned char c_dir[12][8] = {
-    {8, 10, 3, 3, 4, 5, 4, 5}, {2, 2, 11, 9, 4, 5, 4, 5}, {7, 6, 7, 6, 8, 10, 1, 1},
-    {7, 6, 7, 6, 0, 0, 11, 9}, {0, 8, 1, 11, 6, 8, 6, 11}, {10, 0, 9, 1, 10, 7, 9, 7},
-    {10, 4, 9, 4, 10, 2, 9, 3}, {5, 8, 5, 11, 2, 8, 3, 11}, {4, 9, 0, 0, 7, 9, 2, 2},
-    {1, 1, 8, 5, 3, 3, 8, 6}, {11, 5, 0, 0, 11, 6, 2, 2}, {1, 1, 4, 10, 3, 3, 7, 10}};
-__constant__ unsigned char c_hil[12][8] = {
-    {0, 7, 3, 4, 1, 6, 2, 5}, {4, 3, 7, 0, 5, 2, 6, 1}, {6, 1, 5, 2, 7, 0, 4, 3},
-    {2, 5, 1, 6, 3, 4, 0, 7}, {0, 1, 7, 6, 3, 2, 4, 5}, {6, 7, 1, 0, 5, 4, 2, 3},
-    {2, 3, 5, 4, 1, 0, 6, 7}, {4, 5, 3, 2, 7, 6, 0, 1}, {0, 3, 1, 2, 7, 4, 6, 5},
-    {2, 1, 3, 0, 5, 6, 4, 7}, {4, 7, 5, 6, 3, 0, 2, 1}, {6, 5, 7, 4, 1, 2, 0, 3}};
-
-// i = (uint)(MASK * (p - o) / maxSpan) in FP64, truncated (Body.cpp:14-16).
-__device__ inline uint64_t quantise(float p, double o, double s)
-{
-    if (!(s > 0.0)) return 0;
-    double t = kMaskCoord * ((double)p - o);
-    t = t / s;
-    return (uint64_t)t;
-}
-
-// Spreads the low 21 bits of v to every third bit (bit j -> bit 3j).
-__device__ inline uint64_t spread3(uint64_t v)
-{
-    v &= 0x1fffffull;
-    v = (v | (v << 32)) & 0x001f00000000ffffull;
-    v = (v | (v << 16)) & 0x001f0000ff0000ffull;
-    v = (v | (v << 8)) & 0x100f00f00f00f00full;
-    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
-    v = (v | (v << 2)) & 0x1249249249249249ull;
-    return v;
-}
-
-// Body::getKey: 2^63 + sum_j 8^j (x_j + 2 y_j + 4 z_j) (Body.cpp:19-31).
-__device__ inline uint64_t morton_key(float x, float y, float z, const double *ext)
-{
-    uint64_t ix = quantise(x, ext[0], ext[3]);
-    uint64_t iy = quantise(y, ext[1], ext[3]);
-    uint64_t iz = quantise(z, ext[2], ext[3]);
-    return (1ull << 63) | spread3(ix) | (spread3(iy) << 1) | (spread3(iz) << 2);
-}
-
-__device__ inline uint64_t hilbert_of(uint64_t lebesgue)
-{
-    uint64_t h = 0;
-    int dir = 0;
-    for (int lvl = kKeyLevels; lvl > 0; --lvl) {
-        int cell = (int)((lebesgue >> ((lvl - 1) * 3)) & 7u);
-        h = (h << 3) + c_hil[dir][cell];
-        dir = c_dir[dir][cell];
-    }
-    return h;
-}
-
 __global__ void __launch_bounds__(256) keys_kernel(
     const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z, int n,
     const DeviceState *__restrict__ st, int key_type, uint64_t *__restrict__ keys)
@@ -268,6 +213,98 @@ void launch_fill_f32(hipStream_t s, float *p, float v, int64_t n)
 {
     if (n <= 0) return;
     fill_f32_kernel<<<grid_for(n, 256, 4096), 256, 0, s>>>(p, v, n);
+}
+
+// ------------------------------------------------------------ batched form
+// Root cell N + s of every system: empty slots, level 0, no parent.
+__global__ void __launch_bounds__(256) reset_batch_kernel(int S, int N, int *__restrict__ child, int *index,
+                                                          int32_t *__restrict__ cell_level,
+                                                          int32_t *__restrict__ parent,
+                                                          int32_t *__restrict__ arrive, DeviceState *st)
+{
+    const int4 neg = make_int4(-1, -1, -1, -1);
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += blockDim.x * gridDim.x) {
+        const int64_t c = (int64_t)N + s;
+        int4 *c4 = reinterpret_cast<int4 *>(child + 8 * c);
+        c4[0] = neg;
+        c4[1] = neg;
+        cell_level[c] = 0;
+        parent[c] = -1;
+        arrive[c] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *index = N + S;
+        st->err = 0;
+    }
+}
+
+// One block per system: min/max over [off[s], off[s+1]) and the FP64 extent.
+__global__ void __launch_bounds__(256) bbox_batch_kernel(
+    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+    const int32_t *__restrict__ off, float *__restrict__ mm, double *__restrict__ ext, DeviceState *st)
+{
+    __shared__ float red[6][4];
+    const int s = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lo = off[s], hi = off[s + 1];
+    float a = INFINITY, b = -INFINITY, c = INFINITY, d = -INFINITY, e = INFINITY, f = -INFINITY;
+    int bad = 0;
+    for (int i = lo + tid; i < hi; i += blockDim.x) {
+        float xi = x[i], yi = y[i], zi = z[i];
+        bad |= !(isfinite(xi) && isfinite(yi) && isfinite(zi));
+        a = fminf(a, xi); b = fmaxf(b, xi);
+        c = fminf(c, yi); d = fmaxf(d, yi);
+        e = fminf(e, zi); f = fmaxf(f, zi);
+    }
+    if (__ballot(bad) && lane == 0) atomicOr(&st->err, kErrNonFinite);
+    a = wave_min(a); b = wave_max(b); c = wave_min(c); d = wave_max(d); e = wave_min(e); f = wave_max(f);
+    if (lane == 0) {
+        red[0][wave] = a; red[1][wave] = b; red[2][wave] = c;
+        red[3][wave] = d; red[4][wave] = e; red[5][wave] = f;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int nw = blockDim.x >> 6;
+        for (int w = 1; w < nw; ++w) {
+            red[0][0] = fminf(red[0][0], red[0][w]); red[1][0] = fmaxf(red[1][0], red[1][w]);
+            red[2][0] = fminf(red[2][0], red[2][w]); red[3][0] = fmaxf(red[3][0], red[3][w]);
+            red[4][0] = fminf(red[4][0], red[4][w]); red[5][0] = fmaxf(red[5][0], red[5][w]);
+        }
+        for (int k = 0; k < 6; ++k) mm[6 * s + k] = red[k][0];
+        double xs = fabs((double)red[1][0] - (double)red[0][0]);
+        double ys = fabs((double)red[3][0] - (double)red[2][0]);
+        double zs = fabs((double)red[5][0] - (double)red[4][0]);
+        double sp = xs > ys ? xs : ys;
+        sp = sp < zs ? zs : sp;
+        ext[4 * s + 0] = red[0][0]; ext[4 * s + 1] = red[2][0]; ext[4 * s + 2] = red[4][0]; ext[4 * s + 3] = sp;
+    }
+}
+
+__global__ void __launch_bounds__(256) keys_batch_kernel(
+    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z, int n,
+    const int32_t *__restrict__ sys_of, const double *__restrict__ ext, uint64_t *__restrict__ keys)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += blockDim.x * gridDim.x) {
+        const int s = sys_of[i];
+        const double e[4] = {ext[4 * s], ext[4 * s + 1], ext[4 * s + 2], ext[4 * s + 3]};
+        keys[i] = morton_key(x[i], y[i], z[i], e);
+    }
+}
+
+void launch_batch_reset(nbx_batch *b, hipStream_t s)
+{
+    reset_batch_kernel<<<grid_for(b->S, 256, 1024), 256, 0, s>>>(b->S, b->N, b->child, b->index, b->cell_level,
+                                                                 b->parent, b->arrive, b->st);
+}
+
+void launch_batch_bbox(nbx_batch *b, hipStream_t s)
+{
+    bbox_batch_kernel<<<b->S, 256, 0, s>>>(b->x, b->y, b->z, b->off, b->mm, b->ext, b->st);
+}
+
+void launch_batch_keys(nbx_batch *b, hipStream_t s)
+{
+    keys_batch_kernel<<<grid_for(b->N, 256, 65536), 256, 0, s>>>(b->x, b->y, b->z, b->N, b->sys_of, b->ext,
+                                                                 b->keys);
 }
 
 }  // namespace nbx

@@ -130,26 +130,72 @@ __global__ void __launch_bounds__(kSortThreads) radix_downsweep(
     }
 }
 
-void launch_radix_sort(nbx_context *c, hipStream_t s, const uint64_t *keys_in, int n, uint64_t *keys_out,
-                       int32_t *perm_out, uint64_t *tmp_k, int32_t *tmp_v, int32_t *inv_out)
+void launch_radix_passes(uint32_t *hist, hipStream_t s, const uint64_t *keys_in, const int32_t *vals_in, int n,
+                         int first_shift, int npasses, uint64_t *keys_out, int32_t *vals_out, uint64_t *tmp_k,
+                         int32_t *tmp_v, int32_t *inv_out)
 {
-    if (n <= 0) return;
+    if (n <= 0 || npasses <= 0) return;
     const int ntiles = (n + kSortTile - 1) / kSortTile;
-    const int passes = 64 / kRadixBits;  // even: the last pass writes keys_out / perm_out
     const uint64_t *ki = keys_in;
-    const int32_t *vi = nullptr;
-    for (int p = 0; p < passes; ++p) {
-        const int shift = p * kRadixBits;
-        const bool last = (p == passes - 1);
-        uint64_t *ko = (p % 2 == 0) ? tmp_k : keys_out;
-        int32_t *vo = (p % 2 == 0) ? tmp_v : perm_out;
-        radix_upsweep<<<ntiles, kSortThreads, 0, s>>>(ki, n, shift, ntiles, c->hist);
-        radix_scan<<<1, 1024, 0, s>>>(c->hist, ntiles * kRadix);
-        radix_downsweep<<<ntiles, kSortThreads, 0, s>>>(ki, vi, n, shift, ntiles, c->hist, ko, vo,
+    const int32_t *vi = vals_in;
+    for (int p = 0; p < npasses; ++p) {
+        const int shift = first_shift + p * kRadixBits;
+        const bool last = (p == npasses - 1);
+        // alternate so that the last pass lands in keys_out / vals_out
+        const bool to_out = ((npasses - 1 - p) % 2) == 0;
+        uint64_t *ko = to_out ? keys_out : tmp_k;
+        int32_t *vo = to_out ? vals_out : tmp_v;
+        radix_upsweep<<<ntiles, kSortThreads, 0, s>>>(ki, n, shift, ntiles, hist);
+        radix_scan<<<1, 1024, 0, s>>>(hist, ntiles * kRadix);
+        radix_downsweep<<<ntiles, kSortThreads, 0, s>>>(ki, vi, n, shift, ntiles, hist, ko, vo,
                                                         last ? inv_out : nullptr);
         ki = ko;
         vi = vo;
     }
+}
+
+// Batched form: after the 8 key passes, a stable pass over the system index
+// groups every system's bodies, in key order, at [off[s], off[s+1]).
+__global__ void __launch_bounds__(256) system_key_kernel(int n, const int32_t *__restrict__ perm,
+                                                         const int32_t *__restrict__ sys_of,
+                                                         uint64_t *__restrict__ k2)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += blockDim.x * gridDim.x)
+        k2[k] = (uint64_t)sys_of[perm[k]];
+}
+
+__global__ void __launch_bounds__(256) gather_sorted_kernel(int n, const int32_t *__restrict__ perm,
+                                                            const uint64_t *__restrict__ keys,
+                                                            uint64_t *__restrict__ skeys,
+                                                            int32_t *__restrict__ pos)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += blockDim.x * gridDim.x) {
+        const int b = perm[k];
+        skeys[k] = keys[b];
+        pos[b] = k;
+    }
+}
+
+void launch_batch_sort(nbx_batch *b, hipStream_t s)
+{
+    const int n = b->N;
+    int g = (n + 255) / 256;
+    if (g > 65536) g = 65536;
+    // 1. stable sort of (key, body) over the 64 key bits
+    launch_radix_passes(b->hist, s, b->keys, nullptr, n, 0, 64 / kRadixBits, b->skeys, b->perm1, b->kbuf,
+                        b->vbuf, nullptr);
+    // 2. stable sort by system (16 bits cover 65536 systems)
+    system_key_kernel<<<g, 256, 0, s>>>(n, b->perm1, b->sys_of, b->k2);
+    launch_radix_passes(b->hist, s, b->k2, b->perm1, n, 0, 2, b->k2buf, b->perm, b->kbuf, b->vbuf, nullptr);
+    // 3. keys in (system, key) order and the inverse permutation
+    gather_sorted_kernel<<<g, 256, 0, s>>>(n, b->perm, b->keys, b->skeys, b->pos);
+}
+
+void launch_radix_sort(nbx_context *c, hipStream_t s, const uint64_t *keys_in, int n, uint64_t *keys_out,
+                       int32_t *perm_out, uint64_t *tmp_k, int32_t *tmp_v, int32_t *inv_out)
+{
+    launch_radix_passes(c->hist, s, keys_in, nullptr, n, 0, 64 / kRadixBits, keys_out, perm_out, tmp_k, tmp_v,
+                        inv_out);
 }
 
 }  // namespace nbx

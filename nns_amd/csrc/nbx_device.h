@@ -1,0 +1,107 @@
+// nbx_device.h -- device helpers shared by the kernel files (internal linkage,
+// one copy per code object).
+#pragma once
+
+#include "nbx_internal.h"
+
+namespace nbx {
+
+// ---------------------------------------------------------------- keys
+// i = (uint)(MASK * (p - o) / maxSpan) in FP64, truncated
+// (3D/OpenMP/OpenMP_NBody/src/Body.cpp:14-16).  maxSpan == 0 maps to 0.
+static __device__ inline uint64_t quantise(float p, double o, double s)
+{
+    if (!(s > 0.0)) return 0;
+    double t = kMaskCoord * ((double)p - o);
+    t = t / s;
+    return (uint64_t)t;
+}
+
+// Spreads the low 21 bits of v to every third bit (bit j -> bit 3j).
+static __device__ inline uint64_t spread3(uint64_t v)
+{
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x001f00000000ffffull;
+    v = (v | (v << 16)) & 0x001f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+// Body::getKey: 2^63 + sum_j 8^j (x_j + 2 y_j + 4 z_j) (Body.cpp:19-31).
+static __device__ inline uint64_t morton_key(float x, float y, float z, const double *ext)
+{
+    uint64_t ix = quantise(x, ext[0], ext[3]);
+    uint64_t iy = quantise(y, ext[1], ext[3]);
+    uint64_t iz = quantise(z, ext[2], ext[3]);
+    return (1ull << 63) | spread3(ix) | (spread3(iy) << 1) | (spread3(iz) << 2);
+}
+
+// Lebesgue2Hilbert state tables (MolecularDynamics/BarnesHutParallel/
+// SpaceFillingCurves/src/Tree.cpp:71-80); the values define the key.
+static __constant__ unsigned char c_dir[12][8] = {
+    {8, 10, 3, 3, 4, 5, 4, 5}, {2, 2, 11, 9, 4, 5, 4, 5}, {7, 6, 7, 6, 8, 10, 1, 1},
+    {7, 6, 7, 6, 0, 0, 11, 9}, {0, 8, 1, 11, 6, 8, 6, 11}, {10, 0, 9, 1, 10, 7, 9, 7},
+    {10, 4, 9, 4, 10, 2, 9, 3}, {5, 8, 5, 11, 2, 8, 3, 11}, {4, 9, 0, 0, 7, 9, 2, 2},
+    {1, 1, 8, 5, 3, 3, 8, 6}, {11, 5, 0, 0, 11, 6, 2, 2}, {1, 1, 4, 10, 3, 3, 7, 10}};
+static __constant__ unsigned char c_hil[12][8] = {
+    {0, 7, 3, 4, 1, 6, 2, 5}, {4, 3, 7, 0, 5, 2, 6, 1}, {6, 1, 5, 2, 7, 0, 4, 3},
+    {2, 5, 1, 6, 3, 4, 0, 7}, {0, 1, 7, 6, 3, 2, 4, 5}, {6, 7, 1, 0, 5, 4, 2, 3},
+    {2, 3, 5, 4, 1, 0, 6, 7}, {4, 5, 3, 2, 7, 6, 0, 1}, {0, 3, 1, 2, 7, 4, 6, 5},
+    {2, 1, 3, 0, 5, 6, 4, 7}, {4, 7, 5, 6, 3, 0, 2, 1}, {6, 5, 7, 4, 1, 2, 0, 3}};
+
+// Lebesgue2Hilbert(lebesgue, 21) (SFC/src/Tree.cpp:82-104): all 21 digits mapped.
+static __device__ inline uint64_t hilbert_of(uint64_t lebesgue)
+{
+    uint64_t h = 0;
+    int dir = 0;
+    for (int lvl = kKeyLevels; lvl > 0; --lvl) {
+        int cell = (int)((lebesgue >> ((lvl - 1) * 3)) & 7u);
+        h = (h << 3) + c_hil[dir][cell];
+        dir = c_dir[dir][cell];
+    }
+    return h;
+}
+
+// ---------------------------------------------------------------- tree
+static __device__ inline int digit_at(uint64_t key, int level) { return (int)((key >> (60 - 3 * level)) & 7u); }
+
+static __device__ inline int ld_slot(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+static __device__ inline void st_slot(int *p, int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+static __device__ inline int cas_slot(int *p, int expect, int desired)
+{
+    __hip_atomic_compare_exchange_strong(p, &expect, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    return expect;  // value found (== the expected value on success)
+}
+
+static __device__ inline int latched(const DeviceState *st)
+{
+    return __hip_atomic_load(&st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------- launch shape
+// Hardware dispatches consecutive blocks round-robin over the 8 XCDs; this maps
+// physical block b to a logical block so that each XCD gets a contiguous range.
+static __device__ inline int xcd_swizzle(int b, int nblk)
+{
+    const int q = nblk / 8, r = nblk % 8, x = b % 8;
+    return x * q + min(x, r) + b / 8;
+}
+
+static __device__ inline void kahan_add(float &sum, float &comp, float v)
+{
+    const float y = v - comp;
+    const float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+}  // namespace nbx

@@ -71,6 +71,44 @@ struct nbx_context {
     int n = 0, m = 0;
     int stage = 0;  // 0 reset, 1 bbox, 2 built, 3 summarised, 4 sorted, 5 forces
     bool fresh = false;  // arrays reset since the last build (root slots are -1)
+    int force_waves = 4; // waves per block of the force kernel (2, 4 or 8; autotuned)
+};
+
+// Ensemble of S independent systems (batched form).  System s owns bodies
+// [off[s], off[s+1]) of the concatenated arrays and the root cell N + s; the
+// other cells are allocated from N + S upwards.  All arrays are owned by the
+// batch except the bound body arrays.
+struct nbx_batch {
+    nbx_params p{};
+    int device = 0;
+    int S = 0, N = 0, m = 0;
+    int ngroups = 0;
+    std::string last_error;
+    int force_waves = 4;
+    // device
+    nbx::DeviceState *st = nullptr;
+    int32_t *off = nullptr, *sys_of = nullptr;
+    float *theta = nullptr;
+    int4 *groups = nullptr;  // {first sorted position, bodies, system, 0}
+    double *ext = nullptr;   // 4 per system
+    float *mm = nullptr;     // 6 per system
+    uint64_t *keys = nullptr, *skeys = nullptr, *kbuf = nullptr, *k2 = nullptr, *k2buf = nullptr;
+    int32_t *perm = nullptr, *perm1 = nullptr, *vbuf = nullptr, *pos = nullptr, *sorted = nullptr;
+    uint32_t *hist = nullptr;
+    int *child = nullptr, *index = nullptr, *count = nullptr, *start = nullptr;
+    int32_t *cell_level = nullptr, *parent = nullptr, *arrive = nullptr;
+    double *mass64 = nullptr, *mom64 = nullptr;
+    float4 *nodes4 = nullptr;
+    int32_t *tchild = nullptr;
+    int64_t *counts = nullptr;
+    float *acc_scratch = nullptr;
+    hipEvent_t stage_ev[8] = {};
+    // bound body arrays (caller-owned, N each)
+    float *x = nullptr, *y = nullptr, *z = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    float *ax = nullptr, *ay = nullptr, *az = nullptr, *mass = nullptr;
+    // captured step
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace nbx {
@@ -92,6 +130,10 @@ void launch_fill_f32(hipStream_t s, float *p, float v, int64_t n);
 // inv_out when not null.  tmp_k / tmp_v: scratch of n elements.
 void launch_radix_sort(nbx_context *c, hipStream_t s, const uint64_t *keys_in, int n, uint64_t *keys_out,
                        int32_t *perm_out, uint64_t *tmp_k, int32_t *tmp_v, int32_t *inv_out);
+// npasses passes of 8 bits from bit first_shift; vals_in == nullptr means 0, 1, 2, ...
+void launch_radix_passes(uint32_t *hist, hipStream_t s, const uint64_t *keys_in, const int32_t *vals_in, int n,
+                         int first_shift, int npasses, uint64_t *keys_out, int32_t *vals_out, uint64_t *tmp_k,
+                         int32_t *tmp_v, int32_t *inv_out);
 
 // launchers (k_tree.hip)
 void launch_build(nbx_context *c, hipStream_t s, int n, int m, int *child, int *index);
@@ -100,5 +142,16 @@ void launch_layout(nbx_context *c, hipStream_t s, int *sorted);
 
 // launchers (k_force.hip)
 void launch_force(nbx_context *c, hipStream_t s, float *ax, float *ay, float *az, float g, int64_t *counts);
+void launch_force_waves(nbx_context *c, hipStream_t s, float *ax, float *ay, float *az, float g, int waves);
+
+// batched form (k_basic.hip, k_radix.hip, k_tree.hip, k_force.hip)
+void launch_batch_reset(nbx_batch *b, hipStream_t s);
+void launch_batch_bbox(nbx_batch *b, hipStream_t s);
+void launch_batch_keys(nbx_batch *b, hipStream_t s);
+void launch_batch_sort(nbx_batch *b, hipStream_t s);
+void launch_batch_build(nbx_batch *b, hipStream_t s);
+void launch_batch_summarize(nbx_batch *b, hipStream_t s);
+void launch_batch_layout(nbx_batch *b, hipStream_t s);
+void launch_batch_force(nbx_batch *b, hipStream_t s, int64_t *counts, int waves);
 
 }  // namespace nbx
