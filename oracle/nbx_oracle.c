@@ -422,6 +422,111 @@ int orc_tree_accel(const orc_tree *t, double theta, double eps2, int mac_mode,
     return 0;
 }
 
+/* ------------------------------------------------ vector-Jacobian product */
+/* VJP of the accelerations of orc_tree_accel with the interaction lists held
+ * fixed: given u[i] = dL/da_i, returns gx[k] = dL/dx_k and gm[k] = dL/dm_k.
+ * Per accepted node n of body i (d = X_n - x_i, r2 = |d|^2 + eps2, s = r2^-3/2):
+ *   h = M_n s (u_i - 3 (u_i.d) d / r2);  dL/dx_i -= h;  dL/dX_n += h;  dL/dM_n += s (u_i.d).
+ * A leaf passes dL/dX, dL/dM straight to its body.  A cell passes them to every
+ * body k below it through X = sum m x / M, M = sum m:
+ *   dL/dx_k += m_k G_n / M_n,   dL/dm_k += GM_n + G_n.(x_k - X_n) / M_n,
+ * accumulated top-down over the ancestors.  Zero-distance pairs contribute
+ * nothing, as in the forward walk.  Serial; FP64 throughout (cell centres and
+ * masses in FP64 for the chain, in the mode's precision for the interaction). */
+int orc_tree_vjp(const orc_tree *t, double theta, double eps2, int mac_mode, const double *u, double *gx,
+                 double *gm)
+{
+    const int64_t n = t->n, nc = t->ncell;
+    for (int64_t k = 0; k < n; ++k) { gx[3 * k] = gx[3 * k + 1] = gx[3 * k + 2] = 0.0; gm[k] = 0.0; }
+    if (n < 2) return 0;
+    double *G = calloc((size_t)(4 * nc), sizeof(double)); /* Gx, Gy, Gz, GM per cell */
+    const float crit0 = orc_crit0(t->lcell0, theta);
+    float critf[ORC_KEY_LEVELS + 2];
+    double lcell[ORC_KEY_LEVELS + 2];
+    for (int L = 0; L <= ORC_KEY_LEVELS + 1; ++L) {
+        critf[L] = ldexpf(crit0, -2 * L);
+        lcell[L] = ldexp(t->lcell0, -L);
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        const float pxf = t->x[i], pyf = t->y[i], pzf = t->z[i];
+        const double px = pxf, py = pyf, pz = pzf;
+        const double ux = u[3 * i], uy = u[3 * i + 1], uz = u[3 * i + 2];
+        int32_t stk[8 * (ORC_KEY_LEVELS + 2)];
+        int32_t sp = 0;
+        stk[sp++] = -2;
+        while (sp > 0) {
+            int32_t ref = stk[--sp];
+            double mj, qx, qy, qz;
+            int64_t c = -1;
+            if (ref >= 0) {
+                mj = t->m[ref]; qx = t->x[ref]; qy = t->y[ref]; qz = t->z[ref];
+            } else {
+                c = -2 - (int64_t)ref;
+                int L = t->level[c];
+                int accept;
+                if (mac_mode == 0) {
+                    double dx = t->com[3 * c] - px, dy = t->com[3 * c + 1] - py, dz = t->com[3 * c + 2] - pz;
+                    accept = (lcell[L] / sqrt(dx * dx + dy * dy + dz * dz)) < theta;
+                    mj = t->mass[c]; qx = t->com[3 * c]; qy = t->com[3 * c + 1]; qz = t->com[3 * c + 2];
+                } else {
+                    float dx = t->c32[4 * c] - pxf, dy = t->c32[4 * c + 1] - pyf, dz = t->c32[4 * c + 2] - pzf;
+                    accept = critf[L] < fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    mj = t->c32[4 * c + 3]; qx = t->c32[4 * c]; qy = t->c32[4 * c + 1]; qz = t->c32[4 * c + 2];
+                }
+                if (!accept) {
+                    for (int s = 7; s >= 0; --s) {
+                        int32_t ch = t->child[8 * c + s];
+                        if (ch != -1) stk[sp++] = ch;
+                    }
+                    continue;
+                }
+            }
+            double dx = qx - px, dy = qy - py, dz = qz - pz;
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 == 0.0) continue;
+            double r2 = d2 + eps2;
+            double s = 1.0 / (r2 * sqrt(r2));
+            double ud = ux * dx + uy * dy + uz * dz;
+            double k3 = 3.0 * ud / r2;
+            double hx = mj * s * (ux - k3 * dx), hy = mj * s * (uy - k3 * dy), hz = mj * s * (uz - k3 * dz);
+            gx[3 * i] -= hx; gx[3 * i + 1] -= hy; gx[3 * i + 2] -= hz;
+            if (ref >= 0) {
+                gx[3 * ref] += hx; gx[3 * ref + 1] += hy; gx[3 * ref + 2] += hz;
+                gm[ref] += s * ud;
+            } else {
+                G[4 * c] += hx; G[4 * c + 1] += hy; G[4 * c + 2] += hz; G[4 * c + 3] += s * ud;
+            }
+        }
+    }
+    /* top-down: A = sum over ancestors of G/M, B = sum of GM - G.X/M */
+    double *A = malloc(sizeof(double) * 4 * (size_t)nc);
+    int64_t *stk = malloc(sizeof(int64_t) * (size_t)(nc + 1));
+    int64_t sp = 0;
+    stk[sp++] = 0;
+    while (sp > 0) {
+        int64_t c = stk[--sp];
+        const double M = t->mass[c];
+        double a0 = G[4 * c] / M, a1 = G[4 * c + 1] / M, a2 = G[4 * c + 2] / M;
+        double b = G[4 * c + 3] - (a0 * t->com[3 * c] + a1 * t->com[3 * c + 1] + a2 * t->com[3 * c + 2]);
+        int32_t par = t->parent[c];
+        if (par >= 0) { a0 += A[4 * par]; a1 += A[4 * par + 1]; a2 += A[4 * par + 2]; b += A[4 * par + 3]; }
+        A[4 * c] = a0; A[4 * c + 1] = a1; A[4 * c + 2] = a2; A[4 * c + 3] = b;
+        for (int s = 0; s < 8; ++s) {
+            int32_t ch = t->child[8 * c + s];
+            if (ch == -1) continue;
+            if (ch >= 0) {
+                const double mk = t->m[ch];
+                gx[3 * ch] += mk * a0; gx[3 * ch + 1] += mk * a1; gx[3 * ch + 2] += mk * a2;
+                gm[ch] += b + a0 * (double)t->x[ch] + a1 * (double)t->y[ch] + a2 * (double)t->z[ch];
+            } else {
+                stk[sp++] = -2 - (int64_t)ch;
+            }
+        }
+    }
+    free(A); free(stk); free(G);
+    return 0;
+}
+
 /* --------------------------------------------------------- direct sum */
 void orc_direct_accel(int64_t n, const float *x, const float *y, const float *z, const float *m,
                       double eps2, int64_t ntarget, const int32_t *targets, double *acc)

@@ -70,6 +70,8 @@ def lib():
                                  C.c_void_p, _f64p, C.c_void_p]
     L.orc_direct_accel.argtypes = [C.c_int64, _f32p, _f32p, _f32p, _f32p, C.c_double, C.c_int64,
                                    C.c_void_p, _f64p]
+    L.orc_tree_vjp.restype = C.c_int
+    L.orc_tree_vjp.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _f64p, _f64p, _f64p]
     L.orc_direct_accel_fast.argtypes = [C.c_int64, _f32p, _f32p, _f32p, _f32p, C.c_double, C.c_int64,
                                         C.c_void_p, _f64p]
     L.orc_update.argtypes = [C.c_int64] + [_f32p] * 9 + [C.c_float, C.c_float]
@@ -159,8 +161,8 @@ class KeyTree:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            lib().orc_tree_free(h)
+        if h and _lib is not None:
+            _lib.orc_tree_free(h)
             self._h = None
 
     def crit0(self, theta):
@@ -180,6 +182,14 @@ class KeyTree:
         if rc != 0:
             raise RuntimeError("orc_tree_accel failed")
         return acc.reshape(k, 3), cnt
+
+    def vjp(self, theta, eps2, u, mac_mode=0):
+        """dL/dx (n,3) and dL/dm (n) for L = sum_i u_i . a_i with the lists held fixed."""
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        gx = np.zeros(3 * self.n, np.float64)
+        gm = np.zeros(self.n, np.float64)
+        lib().orc_tree_vjp(self._h, float(theta), float(eps2), int(mac_mode), u, gx, gm)
+        return gx.reshape(self.n, 3), gm
 
 
 def direct_accel(x, y, z, m, eps2, targets=None):
@@ -233,8 +243,8 @@ class GeoTree:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            lib().orc_geo_free(h)
+        if h and _lib is not None:
+            _lib.orc_geo_free(h)
             self._h = None
 
     def accel_reference_si(self, theta, softening_m, time_step=1.0):

@@ -270,3 +270,73 @@ def test_update_is_symplectic_euler(oracle):
         np.testing.assert_allclose(vel, ev, rtol=2e-7, atol=1e-9)
         ex = p0.astype(np.float64) + np.float32(0.5 * np.float32(1e-3)) * vel.astype(np.float64)
         np.testing.assert_allclose(pos, ex, rtol=2e-7, atol=1e-9)
+
+
+# ------------------------------------------------------------------ adjoint
+def _direct_sum_autograd(x, y, z, m, eps2, u):
+    import torch
+    X = torch.tensor(np.stack([x, y, z], 1).astype(np.float64), requires_grad=True)
+    M = torch.tensor(m.astype(np.float64), requires_grad=True)
+    d = X[None, :, :] - X[:, None, :]
+    d2 = (d ** 2).sum(-1)
+    w = torch.where(d2 > 0, M[None, :] / (d2 + eps2) ** 1.5, torch.zeros_like(d2))
+    a = (w[:, :, None] * d).sum(1)
+    (torch.tensor(u) * a).sum().backward()
+    return X.grad.numpy(), M.grad.numpy()
+
+
+def test_vjp_theta_zero_is_direct_sum_autograd(oracle):
+    """With every cell opened the walk is the direct sum, and its VJP must be
+    torch's FP64 autograd of the direct sum."""
+    r = np.random.default_rng(21)
+    n = 300
+    x, y, z = (r.normal(0, 1, n).astype(np.float32) for _ in range(3))
+    m = r.uniform(0.5, 1.5, n).astype(np.float32)
+    u = r.normal(0, 1, (n, 3))
+    t = oracle.KeyTree(x, y, z, m)
+    gx, gm = t.vjp(1e-12, 1e-2, u, mac_mode=0)
+    ex, em = _direct_sum_autograd(x, y, z, m, 1e-2, u)
+    np.testing.assert_allclose(gx, ex, rtol=0, atol=1e-12 * np.abs(ex).max())
+    np.testing.assert_allclose(gm, em, rtol=0, atol=1e-12 * np.abs(em).max())
+
+
+def test_vjp_two_clusters_central_differences(oracle):
+    """Two clusters far apart see each other as one accepted cell, so the VJP
+    goes through the centre-of-mass chain; central differences of
+    L = sum u.a (tree rebuilt for each perturbed input) must agree."""
+    r = np.random.default_rng(22)
+    a = r.normal(0, 0.5, (12, 3)) + [-50.0, 0, 0]
+    b = r.normal(0, 0.5, (12, 3)) + [50.0, 3.0, -2.0]
+    p = np.vstack([a, b]).astype(np.float32)
+    m = r.uniform(0.5, 2.0, 24).astype(np.float32)
+    u = r.normal(0, 1, (24, 3))
+    eps2, theta = 0.1, 0.5
+    t = oracle.KeyTree(p[:, 0], p[:, 1], p[:, 2], m)
+    gx, gm = t.vjp(theta, eps2, u, mac_mode=0)
+    _, cnt0 = t.accel(theta, eps2, mac_mode=0)
+    assert np.all(cnt0 < 23)  # some cells accepted
+
+    def loss(pp, mm):
+        tt = oracle.KeyTree(pp[:, 0], pp[:, 1], pp[:, 2], mm)
+        acc, cnt = tt.accel(theta, eps2, mac_mode=0)
+        return float((u * acc).sum()), np.array_equal(cnt, cnt0)
+
+    checked = 0
+    for k in range(24):
+        for ax in range(3):
+            pp, pm = p.copy(), p.copy()
+            pp[k, ax] += np.float32(2e-3)
+            pm[k, ax] -= np.float32(2e-3)
+            delta = float(pp[k, ax]) - float(pm[k, ax])
+            (lp, okp), (lm, okm) = loss(pp, m), loss(pm, m)
+            if not (okp and okm):
+                continue  # an interaction list changed: not differentiable here
+            checked += 1
+            assert (lp - lm) / delta == pytest.approx(gx[k, ax], rel=1e-3, abs=1e-6 * np.abs(gx).max()), (k, ax)
+        mp, mm_ = m.copy(), m.copy()
+        mp[k] *= np.float32(1.001)
+        mm_[k] *= np.float32(0.999)
+        (lp, _), (lm, _) = loss(p, mp), loss(p, mm_)
+        assert (lp - lm) / (float(mp[k]) - float(mm_[k])) == pytest.approx(gm[k], rel=1e-4,
+                                                                            abs=1e-8 * np.abs(gm).max()), k
+    assert checked >= 54
