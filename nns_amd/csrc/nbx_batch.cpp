@@ -247,16 +247,25 @@ int nbx_batch_autotune(nbx_batch *b, void *stream, int *waves_out)
     const int cand[3] = {2, 4, 8};
     float best = 1e30f;
     int pick = b->force_waves;
+    // time into scratch so that the bound accelerations are left untouched
+    float *ax = b->ax, *ay = b->ay, *az = b->az;
+    b->ax = b->acc_scratch;
+    b->ay = b->acc_scratch + b->N;
+    b->az = b->acc_scratch + 2 * (size_t)b->N;
     for (int w : cand) {
         nbx::launch_batch_force(b, s, nullptr, w);  // warm-up
         hipEventRecord(b->stage_ev[0], s);
         for (int r = 0; r < 3; ++r) nbx::launch_batch_force(b, s, nullptr, w);
         hipEventRecord(b->stage_ev[1], s);
-        if (hipEventSynchronize(b->stage_ev[1]) != hipSuccess) return bfail(b, NBX_ERR_HIP, "hipEventSynchronize");
+        if (hipEventSynchronize(b->stage_ev[1]) != hipSuccess) {
+            b->ax = ax; b->ay = ay; b->az = az;
+            return bfail(b, NBX_ERR_HIP, "hipEventSynchronize");
+        }
         float ms = 0.f;
         hipEventElapsedTime(&ms, b->stage_ev[0], b->stage_ev[1]);
         if (ms < best) { best = ms; pick = w; }
     }
+    b->ax = ax; b->ay = ay; b->az = az;
     b->force_waves = pick;
     drop_graph(b);
     if (waves_out) *waves_out = pick;
