@@ -191,6 +191,11 @@ void launch_batch_sort(nbx_batch *b, hipStream_t s)
     gather_sorted_kernel<<<g, 256, 0, s>>>(n, b->perm, b->keys, b->skeys, b->pos);
 }
 
+void launch_exclusive_scan_u32(uint32_t *data, int total, hipStream_t s)
+{
+    if (total > 0) radix_scan<<<1, 1024, 0, s>>>(data, total);
+}
+
 void launch_radix_sort(nbx_context *c, hipStream_t s, const uint64_t *keys_in, int n, uint64_t *keys_out,
                        int32_t *perm_out, uint64_t *tmp_k, int32_t *tmp_v, int32_t *inv_out)
 {

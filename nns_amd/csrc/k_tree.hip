@@ -1,146 +1,361 @@
-// k_tree.hip -- tree construction (atomic child-slot locking), bottom-up summary,
-// and the traversal layout.
+// k_tree.hip -- tree construction, bottom-up summary and the traversal layout.
 //
 // Cells are prefixes of the 63-bit Morton key.  A cell at level L selects the
 // slot of a body by key digit L (bits 62-3L .. 60-3L), so slot order is key
-// order.  Single system: the root is node 0 with its slots at child[0..7] (the
+// order and the bodies of any cell are a contiguous range of the sorted keys.
+// Single system: the root is node 0 with its slots at child[0..7] (the
 // reference's convention, src/Kernels.cu:195,225) and cells take ids n, n+1, ...
-// from *index.  Ensemble: system s has the root cell N + s, other cells follow.
+// Ensemble: system s has the root cell N + s and cells follow from N + S.
+//
+// Construction is split along the MI355X memory hierarchy.  Each XCD has its
+// own L2, so a fence or atomic at device scope costs a trip past L2; those are
+// kept to a handful per subtree:
+//   1. top cells -- cells holding more than kSubtreeMax bodies -- are found
+//      from the sorted keys by binary search (one thread per sample of
+//      kSubtreeMax bodies) and wired with their child ranges;
+//   2. every child range of a top cell with 2..kSubtreeMax bodies is a subtree
+//      built by ONE workgroup with the lock-based insertion (compare-and-swap
+//      on child slots, -2 = locked) at workgroup scope; its exact cell count
+//      comes from the common-prefix lengths of neighbouring keys, so one
+//      atomicAdd reserves its contiguous block of cell ids;
+//   3. the summary runs level by level: subtrees inside their workgroup, then
+//      the top cells in one block.  Every cell is summed by one thread over its
+//      children in slot order, in FP64: results do not depend on scheduling.
 #include "nbx_device.h"
+
+#include <algorithm>
 
 namespace nbx {
 
-// Lock-based insertion of one body per thread, bodies taken in key order.
-//
-// Each iteration a lane reads the slot for its body at the current cell
-// (relaxed agent-scope atomic loads; L2 is the coherence point):
-//   cell id (>= n)  descend;
-//   -1              CAS -1 -> body, done;
-//   -2              locked by another lane, retry;
-//   body            CAS body -> -2 (lock); the winner creates the chain of
-//                   cells down to the first level where the two keys differ
-//                   (a level-21 bucket if the keys are equal), fences, and
-//                   publishes the first cell with an atomic store (unlock).
-// No lane ever waits inside an iteration on another lane, so a wave whose
-// lanes contend for one slot makes progress: the CAS winner finishes its
-// insertion in the same iteration.  Cell ids for a wave's splits come from
-// one atomicAdd on *index (wave prefix sum).
-template <bool BATCH>
-__global__ void __launch_bounds__(256) build_kernel(
-    int n, int m, const uint64_t *__restrict__ skeys, const int32_t *__restrict__ perm,
-    const uint64_t *__restrict__ keys, int *child, int *index, int32_t *__restrict__ cell_level,
-    int32_t *__restrict__ parent, int32_t *__restrict__ arrive, DeviceState *st,
-    const int32_t *__restrict__ sys_of, int rootbase)
+// Segments: one system ([0, n), root node 0) or an ensemble (off[], roots n + s).
+struct Segs {
+    const int32_t *off;       // nullptr: one segment [0, n)
+    const int32_t *samp_off;  // nullptr: its samples start at 0
+    int nseg, n;
+    int batch;                // roots at n + s instead of node 0
+};
+
+__device__ inline int seg_lo(const Segs &g, int s) { return g.off ? g.off[s] : 0; }
+__device__ inline int seg_hi(const Segs &g, int s) { return g.off ? g.off[s + 1] : g.n; }
+__device__ inline int samp_lo(const Segs &g, int s) { return g.samp_off ? g.samp_off[s] : 0; }
+__device__ inline int seg_root(const Segs &g, int s) { return g.batch ? g.n + s : 0; }
+__device__ inline int seg_of_sample(const Segs &g, int k)
+{
+    if (!g.samp_off) return 0;
+    int lo = 0, hi = g.nseg;  // last s with samp_off[s] <= k
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (g.samp_off[mid] <= k) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline uint64_t prefix_of(uint64_t key, int L) { return key >> (63 - 3 * L); }
+
+// First index in [lo, hi) whose level-L prefix is >= P (keys sorted).
+__device__ inline int lower_prefix(const uint64_t *k, int lo, int hi, uint64_t P, int L)
+{
+    while (lo < hi) {
+        int mid = lo + ((hi - lo) >> 1);
+        if (prefix_of(k[mid], L) < P) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+__device__ inline int upper_prefix(const uint64_t *k, int lo, int hi, uint64_t P, int L)
+{
+    while (lo < hi) {
+        int mid = lo + ((hi - lo) >> 1);
+        if (prefix_of(k[mid], L) <= P) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+// First index in [lo, hi) whose digit L is >= d (all keys share the first L digits).
+__device__ inline int lower_digit(const uint64_t *k, int lo, int hi, int d, int L)
+{
+    while (lo < hi) {
+        int mid = lo + ((hi - lo) >> 1);
+        if (digit_at(k[mid], L) < d) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// lcp[k] = number of leading key digits shared by sorted keys k and k+1 (21 if equal).
+__global__ void __launch_bounds__(256) lcp_kernel(int n, const uint64_t *__restrict__ skeys,
+                                                  uint8_t *__restrict__ lcp, DeviceState *st)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->nsub = 0;
+        st->work = 0;
+        st->work2 = 0;
+        st->ntop = 0;
+    }
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n - 1; k += blockDim.x * gridDim.x) {
+        const uint64_t x = (skeys[k] ^ skeys[k + 1]) & ~(1ull << 63);
+        lcp[k] = x == 0 ? (uint8_t)kKeyLevels : (uint8_t)((62 - (63 - __clzll((long long)x))) / 3);
+    }
+}
+
+// Top cells: thread per sample q (every kSubtreeMax-th sorted position of a
+// segment).  The level-L cell through q is [a, b); it is a top cell when it
+// holds more than kSubtreeMax bodies, and q owns it when no earlier sample lies
+// in it.
+__global__ void __launch_bounds__(256) top_cells_kernel(Segs g, int nsamples, const uint64_t *__restrict__ skeys,
+                                                        uint32_t *__restrict__ flag, uint32_t *__restrict__ id,
+                                                        int32_t *__restrict__ ta, int32_t *__restrict__ tb,
+                                                        const DeviceState *st)
 {
     if (latched(st)) return;
-    const int lane = threadIdx.x & 63;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    bool done = p >= n;
-    const int b = done ? 0 : perm[p];
-    const uint64_t key = done ? 0ull : skeys[p];
-    int node = (BATCH && !done) ? rootbase + sys_of[b] : 0;
-    int L = 0;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nsamples) return;
+    for (int L = 0; L < kLevels; ++L) {
+        flag[k * kLevels + L] = 0u;
+        id[k * kLevels + L] = 0u;
+    }
+    const int s = seg_of_sample(g, k);
+    const int lo = seg_lo(g, s), hi = seg_hi(g, s);
+    const int q = lo + (k - samp_lo(g, s)) * kSubtreeMax;
+    const uint64_t key = skeys[q];
+    int a = lo, b = hi;
+    for (int L = 1; L < kLevels; ++L) {
+        const uint64_t P = prefix_of(key, L);
+        a = lower_prefix(skeys, a, q + 1, P, L);
+        b = upper_prefix(skeys, q, b, P, L);
+        if (b - a <= kSubtreeMax) break;
+        if (q - kSubtreeMax < a || q == lo) {
+            flag[k * kLevels + L] = 1u;
+            id[k * kLevels + L] = 1u;
+            ta[k * kLevels + L] = a;
+            tb[k * kLevels + L] = b;
+        }
+    }
+}
+
+// Wires every top cell (and every root): child ranges by digit L; a range of
+// one body is a leaf, of more than kSubtreeMax a top cell, otherwise a subtree
+// queued for the workgroup build.  Thread 0 also sets *index past the top cells.
+__global__ void __launch_bounds__(256) top_wire_kernel(
+    Segs g, int nsamples, const uint64_t *__restrict__ skeys, const int32_t *__restrict__ perm, int *child,
+    int32_t *__restrict__ cell_level, const uint32_t *__restrict__ flag, const uint32_t *__restrict__ id,
+    const int32_t *__restrict__ ta, const int32_t *__restrict__ tb, int4 *__restrict__ sub, int sub_max,
+    int base0, int m, int *index, DeviceState *st)
+{
+    if (latched(st)) return;
+    const int total = nsamples * kLevels;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        const int ntop = (int)(id[total - 1] + flag[total - 1]);
+        st->ntop = ntop;
+        *index = base0 + ntop;
+        if ((int64_t)base0 + ntop > m) atomicOr(&st->err, kErrCapacity);
+    }
+    int c, L, a, b, s;
+    if (t < total) {
+        if (!flag[t]) return;
+        const int k = t / kLevels;
+        L = t - k * kLevels;
+        c = base0 + (int)id[t];
+        a = ta[t];
+        b = tb[t];
+        s = seg_of_sample(g, k);
+        if (c >= m) return;  // capacity error latched by thread 0
+        cell_level[c] = L;
+        if (L == kKeyLevels) {  // more than kSubtreeMax bodies on one key
+            atomicOr(&st->err, kErrBucket);
+            for (int d = 0; d < 8; ++d) child[8 * (int64_t)c + d] = -1;
+            return;
+        }
+    } else if (t < total + g.nseg) {
+        s = t - total;
+        c = seg_root(g, s);
+        L = 0;
+        a = seg_lo(g, s);
+        b = seg_hi(g, s);
+    } else {
+        return;
+    }
+    const int lo = seg_lo(g, s);
+    int da = a;
+    for (int d = 0; d < 8; ++d) {
+        const int db = lower_digit(skeys, da, b, d + 1, L);
+        const int cnt = db - da;
+        const int64_t slot = 8 * (int64_t)c + d;
+        if (cnt == 0) {
+            child[slot] = -1;
+        } else if (cnt == 1) {
+            child[slot] = perm[da];
+        } else if (cnt > kSubtreeMax) {
+            const int kk = samp_lo(g, s) + (da - lo + kSubtreeMax - 1) / kSubtreeMax;
+            child[slot] = base0 + (int)id[kk * kLevels + L + 1];
+        } else {
+            child[slot] = -1;  // set by the subtree build
+            const int i = atomicAdd(&st->nsub, 1);
+            if (i < sub_max) sub[i] = make_int4(da, db, L + 1, (int)slot);
+            else atomicOr(&st->err, kErrCapacity);
+        }
+        da = db;
+    }
+}
+
+__device__ inline int ld_slot_wg(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline void st_slot_wg(int *p, int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline int cas_slot_wg(int *p, int expect, int desired)
+{
+    __hip_atomic_compare_exchange_strong(p, &expect, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+    return expect;
+}
+
+__device__ inline int block_sum(int v, int *red)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    int s = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
+    __syncthreads();
+    return s;
+}
+
+// Subtree build: persistent workgroups take subtrees from the queue.  Inside a
+// subtree, the bodies are inserted by lock-based descent along their key digits
+// (one body per thread per round, bodies strided over the range so that the
+// lanes of a wave start in different branches):
+//   cell (>= n)  descend;   -1  CAS -1 -> body;   -2  locked, retry next iteration;
+//   body         CAS body -> -2 (lock); the winner creates the chain of cells down
+//                to the first level where the two keys differ (a level-21 bucket if
+//                they are equal), then publishes the first cell with a release store
+//                (unlock).
+// No lane waits inside an iteration on another lane, so a wave whose lanes
+// contend for one slot always makes progress.  All atomics and fences are at
+// workgroup scope: only this workgroup touches the subtree.
+__global__ void __launch_bounds__(256) subtree_build_kernel(
+    int n, int m, const uint64_t *__restrict__ skeys, const int32_t *__restrict__ perm,
+    const uint64_t *__restrict__ keys, const uint8_t *__restrict__ lcp, int *child, int *index,
+    int32_t *__restrict__ cell_level, int4 *__restrict__ sub, int2 *__restrict__ subcells, DeviceState *st)
+{
+    __shared__ int s_job, s_base, s_next, s_bad;
+    __shared__ int red[4];
+    if (latched(st)) return;
+    const int tid = threadIdx.x;
+    const int nsub = __hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
-        int need = 0, addr = 0, found = 0, ld = 0;
-        uint64_t key2 = 0;
-        if (!done) {
-            if (L == kKeyLevels) {
-                // bucket cell: first free slot
-                bool placed = false, full = true;
-                for (int s = 0; s < 8 && !placed; ++s) {
-                    int ch = ld_slot(&child[8 * (int64_t)node + s]);
-                    if (ch == -1) {
-                        full = false;
-                        if (cas_slot(&child[8 * (int64_t)node + s], -1, b) == -1) placed = true;
-                    }
-                }
-                if (placed) {
-                    done = true;
-                } else if (full) {
-                    atomicOr(&st->err, kErrBucket);
-                    done = true;
-                }
+        if (tid == 0) s_job = atomicAdd(&st->work, 1);
+        __syncthreads();
+        const int job = s_job;
+        __syncthreads();
+        if (job >= nsub) break;
+        const int4 jb = sub[job];
+        const int a = jb.x, b = jb.y, L0 = jb.z, slot = jb.w;
+        // exact cell count of the subtree from the common-prefix lengths
+        int contrib = 0;
+        for (int k = a + tid; k < b - 1; k += blockDim.x)
+            contrib += (k == a) ? (int)lcp[a] - L0 + 1 : max(0, (int)lcp[k] - (int)lcp[k - 1]);
+        const int ks = block_sum(contrib, red);
+        if (tid == 0) {
+            const int base = atomicAdd(index, ks);
+            s_bad = (int64_t)base + ks > (int64_t)m;
+            if (s_bad) atomicOr(&st->err, kErrCapacity);
+            s_base = base;
+            s_next = 1;
+            if (!s_bad) {
+                for (int d = 0; d < 8; ++d) child[8 * (int64_t)base + d] = -1;
+                cell_level[base] = L0;
+                child[slot] = base;
+                subcells[job] = make_int2(base, ks);
             } else {
-                addr = 8 * node + digit_at(key, L);
-                const int ch = ld_slot(&child[addr]);
+                subcells[job] = make_int2(0, 0);
+            }
+        }
+        __syncthreads();
+        if (s_bad) continue;
+        const int base = s_base, root = base;
+        const int count = b - a;
+        const int rounds = (count + blockDim.x - 1) / blockDim.x;
+        for (int r = 0; r < rounds; ++r) {
+            const int off = r + tid * rounds;
+            bool done = off >= count;
+            const int p = a + off;
+            const int body = done ? 0 : perm[p];
+            const uint64_t key = done ? 0ull : skeys[p];
+            int node = root, L = L0;
+            while (__ballot(!done)) {
+                if (done) continue;
+                if (L == kKeyLevels) {
+                    bool placed = false, full = true;
+                    for (int s = 0; s < 8 && !placed; ++s) {
+                        int ch = ld_slot_wg(&child[8 * (int64_t)node + s]);
+                        if (ch == -1) {
+                            full = false;
+                            if (cas_slot_wg(&child[8 * (int64_t)node + s], -1, body) == -1) placed = true;
+                        }
+                    }
+                    if (placed) {
+                        done = true;
+                    } else if (full) {
+                        atomicOr(&st->err, kErrBucket);
+                        done = true;
+                    }
+                    continue;
+                }
+                const int64_t addr = 8 * (int64_t)node + digit_at(key, L);
+                const int ch = ld_slot_wg(&child[addr]);
                 if (ch >= n) {
                     node = ch;
                     ++L;
                 } else if (ch == -1) {
-                    if (cas_slot(&child[addr], -1, b) == -1) done = true;
+                    if (cas_slot_wg(&child[addr], -1, body) == -1) done = true;
                 } else if (ch >= 0) {
-                    if (cas_slot(&child[addr], ch, -2) == ch) {
-                        found = ch;
-                        key2 = keys[ch];
-                        ld = L + 1;
+                    if (cas_slot_wg(&child[addr], ch, -2) == ch) {
+                        const uint64_t key2 = keys[ch];
+                        int ld = L + 1;
                         while (ld < kKeyLevels && digit_at(key, ld) == digit_at(key2, ld)) ++ld;
-                        need = ld - L;  // cells at levels L+1 .. ld
-                    }
-                }
-            }
-        }
-        const uint64_t wants = __ballot(need > 0);
-        if (wants) {
-            // inclusive prefix sum of `need` over the wave
-            int incl = need;
-            for (int o = 1; o < 64; o <<= 1) {
-                int v = __shfl_up(incl, o);
-                if (lane >= o) incl += v;
-            }
-            const int total = __shfl(incl, 63);
-            const int leader = __ffsll((unsigned long long)wants) - 1;
-            int base = 0;
-            if (lane == leader) base = atomicAdd(index, total);
-            base = __shfl(base, leader);
-            if (need > 0) {
-                const int first = base + incl - need;
-                if ((int64_t)first + need > (int64_t)m) {
-                    atomicOr(&st->err, kErrCapacity);
-                    st_slot(&child[addr], found);  // unlock, tree stays consistent
-                    done = true;
-                } else {
-                    const int4 neg = make_int4(-1, -1, -1, -1);
-                    for (int j = 0; j < need; ++j) {
-                        const int c = first + j;
-                        const int lvl = L + 1 + j;
-                        int4 *c4 = reinterpret_cast<int4 *>(child + 8 * (int64_t)c);
-                        c4[0] = neg;
-                        c4[1] = neg;
-                        cell_level[c] = lvl;
-                        parent[c] = (j == 0) ? node : c - 1;
-                        arrive[c] = 0;
-                        if (lvl < ld) {
-                            child[8 * c + digit_at(key, lvl)] = c + 1;
-                        } else if (ld < kKeyLevels) {
-                            child[8 * c + digit_at(key2, ld)] = found;
-                            child[8 * c + digit_at(key, ld)] = b;
-                        } else {
-                            child[8 * c + 0] = found;  // bucket; put in index order later
-                            child[8 * c + 1] = b;
+                        const int need = ld - L;  // cells at levels L+1 .. ld
+                        const int old = atomicAdd(&s_next, need);
+                        if (old + need > ks) {
+                            atomicOr(&st->err, kErrCapacity);
+                            st_slot_wg(&child[addr], ch);
+                            done = true;
+                            continue;
                         }
+                        const int first = base + old;
+                        const int4 neg = make_int4(-1, -1, -1, -1);
+                        for (int j = 0; j < need; ++j) {
+                            const int c = first + j;
+                            const int lvl = L + 1 + j;
+                            int4 *c4 = reinterpret_cast<int4 *>(child + 8 * (int64_t)c);
+                            c4[0] = neg;
+                            c4[1] = neg;
+                            cell_level[c] = lvl;
+                            if (lvl < ld) {
+                                child[8 * (int64_t)c + digit_at(key, lvl)] = c + 1;
+                            } else if (ld < kKeyLevels) {
+                                child[8 * (int64_t)c + digit_at(key2, ld)] = ch;
+                                child[8 * (int64_t)c + digit_at(key, ld)] = body;
+                            } else {
+                                child[8 * (int64_t)c + 0] = ch;  // bucket; index order restored in the summary
+                                child[8 * (int64_t)c + 1] = body;
+                            }
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                        st_slot_wg(&child[addr], first);  // publish + unlock
+                        done = true;
                     }
-                    __threadfence();
-                    st_slot(&child[addr], first);  // publish + unlock
-                    done = true;
                 }
             }
         }
-        if (__ballot(!done) == 0) break;
+        __syncthreads();
     }
 }
 
-// Bottom-up summary.  A cell without cell children is summed by its own
-// thread; a cell with k cell children is summed by the thread whose child
-// arrives k-th (atomic counter, release fence before the increment, acquire
-// fence after).  Every cell is therefore summed by exactly one thread, over its
-// children in slot order, in FP64: results do not depend on scheduling.
-__device__ inline int count_cell_children(const int *child, int node, int n)
-{
-    int k = 0;
-    for (int s = 0; s < 8; ++s) k += child[8 * (int64_t)node + s] >= n;
-    return k;
-}
-
+// ------------------------------------------------------------------ summary
 struct CellSum {
     double ms, mx, my, mz;
     int cnt, first;
@@ -187,82 +402,87 @@ __device__ inline CellSum sum_cell(int c, bool bucket, int n, int *child, const 
     return r;
 }
 
-// Single system.  Thread t < C handles cell n + t, thread C the root (C = *index - n).
-__global__ void __launch_bounds__(256) summarize_kernel(
-    int n, const int *__restrict__ index, int *child, const float *x, const float *y, const float *z,
-    float *mass, float *cx, float *cy, float *cz, int *count, int *start, const int32_t *__restrict__ pos,
-    const int32_t *__restrict__ parent, int32_t *arrive, const int32_t *__restrict__ cell_level,
-    double *mass64, double *mom64, float4 *nodes4, DeviceState *st)
+struct SumOut {
+    int n;
+    int *child;
+    const float *x, *y, *z;  // bodies
+    float *mass;             // bodies read, cells written (single system)
+    float *cx, *cy, *cz;     // cell centre of mass (single system; nullptr for the ensemble)
+    int *count, *start;
+    const int32_t *pos, *cell_level;
+    double *mass64, *mom64;
+    float4 *nodes4;
+};
+
+__device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool single_root)
 {
+    const bool bucket = !single_root && o.cell_level[c] == kKeyLevels;
+    const CellSum r = sum_cell(c, bucket, o.n, o.child, o.x, o.y, o.z, o.mass, o.pos, o.mass64, o.mom64, o.count,
+                               o.start);
+    if (r.cnt > 0 && !(r.ms > 0.0)) atomicOr(&st->err, kErrNonFinite);
+    const double gx = r.mx / r.ms, gy = r.my / r.ms, gz = r.mz / r.ms;
+    const float fx = (float)gx, fy = (float)gy, fz = (float)gz, fm = (float)r.ms;
+    if (single_root) {
+        st->root64[0] = r.ms; st->root64[1] = gx; st->root64[2] = gy; st->root64[3] = gz;
+        st->root4 = make_float4(fx, fy, fz, fm);
+        return;
+    }
+    o.mass64[c] = r.ms;
+    o.mom64[3 * (int64_t)c + 0] = r.mx; o.mom64[3 * (int64_t)c + 1] = r.my; o.mom64[3 * (int64_t)c + 2] = r.mz;
+    if (o.cx) {
+        o.cx[c] = fx; o.cy[c] = fy; o.cz[c] = fz; o.mass[c] = fm;
+    }
+    o.count[c] = r.cnt;
+    o.start[c] = r.first;
+    o.nodes4[c] = make_float4(fx, fy, fz, fm);
+}
+
+// Subtree summary: persistent workgroups take subtrees from the queue; the
+// cells of a subtree are the contiguous ids [base, base + ks), summed deepest
+// level first with a barrier between levels.
+__global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const int4 *__restrict__ sub,
+                                                              const int2 *__restrict__ subcells, DeviceState *st)
+{
+    __shared__ int s_job;
     if (latched(st)) return;
-    const int ncell = *index - n;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t > ncell) return;
-    int c = (t < ncell) ? n + t : 0;
-    if (count_cell_children(child, c, n) != 0) return;
+    const int tid = threadIdx.x;
+    const int nsub = __hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
-        const bool bucket = c != 0 && cell_level[c] == kKeyLevels;
-        const CellSum r = sum_cell(c, bucket, n, child, x, y, z, mass, pos, mass64, mom64, count, start);
-        if (!(r.ms > 0.0)) atomicOr(&st->err, kErrNonFinite);
-        const double gx = r.mx / r.ms, gy = r.my / r.ms, gz = r.mz / r.ms;
-        const float fx = (float)gx, fy = (float)gy, fz = (float)gz, fm = (float)r.ms;
-        if (c == 0) {
-            st->root64[0] = r.ms; st->root64[1] = gx; st->root64[2] = gy; st->root64[3] = gz;
-            st->root4 = make_float4(fx, fy, fz, fm);
-            break;
+        if (tid == 0) s_job = atomicAdd(&st->work2, 1);
+        __syncthreads();
+        const int job = s_job;
+        __syncthreads();
+        if (job >= nsub) break;
+        const int2 sc = subcells[job];
+        const int L0 = sub[job].z;
+        const int base = sc.x, ks = sc.y;
+        for (int L = kKeyLevels; L >= L0; --L) {
+            for (int c = base + tid; c < base + ks; c += blockDim.x)
+                if (o.cell_level[c] == L) summarise(o, c, st, false);
+            __syncthreads();
         }
-        mass64[c] = r.ms;
-        mom64[3 * (int64_t)c + 0] = r.mx; mom64[3 * (int64_t)c + 1] = r.my; mom64[3 * (int64_t)c + 2] = r.mz;
-        cx[c] = fx; cy[c] = fy; cz[c] = fz; mass[c] = fm;
-        count[c] = r.cnt;
-        start[c] = r.first;
-        nodes4[c] = make_float4(fx, fy, fz, fm);
-        const int p = parent[c];
-        __threadfence();
-        const int old = atomicAdd(&arrive[p], 1);
-        if (old != count_cell_children(child, p, n) - 1) break;
-        __threadfence();
-        c = p;
     }
 }
 
-// Ensemble.  Thread t handles cell n + t of [n, *index); root cells (parent -1)
-// end the walk up.
-__global__ void __launch_bounds__(256) summarize_batch_kernel(
-    int n, const int *__restrict__ index, int *child, const float *__restrict__ x, const float *__restrict__ y,
-    const float *__restrict__ z, const float *__restrict__ mass, int *count, int *start,
-    const int32_t *__restrict__ pos, const int32_t *__restrict__ parent, int32_t *arrive,
-    const int32_t *__restrict__ cell_level, double *mass64, double *mom64, float4 *nodes4, DeviceState *st)
+// Top summary in one block: top cells deepest level first, then the roots.
+__global__ void __launch_bounds__(1024) top_summary_kernel(SumOut o, Segs g, int nsamples,
+                                                           const uint32_t *__restrict__ flag,
+                                                           const uint32_t *__restrict__ id, int base0,
+                                                           DeviceState *st)
 {
     if (latched(st)) return;
-    const int ncell = *index - n;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ncell) return;
-    int c = n + t;
-    if (count_cell_children(child, c, n) != 0) return;
-    for (;;) {
-        const CellSum r = sum_cell(c, cell_level[c] == kKeyLevels, n, child, x, y, z, mass, pos, mass64, mom64,
-                                   count, start);
-        if (r.cnt > 0 && !(r.ms > 0.0)) atomicOr(&st->err, kErrNonFinite);
-        const double gx = r.mx / r.ms, gy = r.my / r.ms, gz = r.mz / r.ms;
-        mass64[c] = r.ms;
-        mom64[3 * (int64_t)c + 0] = r.mx; mom64[3 * (int64_t)c + 1] = r.my; mom64[3 * (int64_t)c + 2] = r.mz;
-        count[c] = r.cnt;
-        start[c] = r.first;
-        nodes4[c] = make_float4((float)gx, (float)gy, (float)gz, (float)r.ms);
-        const int p = parent[c];
-        if (p < 0) break;
-        __threadfence();
-        const int old = atomicAdd(&arrive[p], 1);
-        if (old != count_cell_children(child, p, n) - 1) break;
-        __threadfence();
-        c = p;
+    for (int L = kKeyLevels; L >= 1; --L) {
+        for (int k = threadIdx.x; k < nsamples; k += blockDim.x)
+            if (flag[k * kLevels + L]) summarise(o, base0 + (int)id[k * kLevels + L], st, false);
+        __syncthreads();
     }
+    for (int s = threadIdx.x; s < g.nseg; s += blockDim.x) summarise(o, seg_root(g, s), st, !g.batch);
 }
 
-// Traversal layout: bodies in key order as float4 (x, y, z, m) at nodes4[0..n),
-// and child references with bodies renamed to their sorted positions
-// (tchild[8c+s]); cells keep their ids.  Also writes sorted[].
+// ------------------------------------------------------------------ layout
+// Bodies in key order as float4 (x, y, z, m) at nodes4[0..n), and child
+// references with bodies renamed to their sorted positions (tchild[8c+s]);
+// cells keep their ids.  Also writes sorted[].
 __global__ void __launch_bounds__(256) layout_bodies_kernel(
     int n, const int32_t *__restrict__ perm, const float *__restrict__ x, const float *__restrict__ y,
     const float *__restrict__ z, const float *__restrict__ mass, float4 *__restrict__ nodes4,
@@ -296,6 +516,7 @@ __global__ void __launch_bounds__(256) layout_child_kernel(
     }
 }
 
+// ------------------------------------------------------------------ launchers
 static inline int capped_grid(int64_t work, int cap = 65536)
 {
     int64_t g = (work + 255) / 256;
@@ -303,19 +524,71 @@ static inline int capped_grid(int64_t work, int cap = 65536)
     return (int)(g > cap ? cap : g);
 }
 
+constexpr int kPersistentBlocks = 2048;  // 256 CUs x 8 workgroups
+
+static void build_tree(Segs g, int nsamples, int base0, int m, TreeScratch &ts, DeviceState *st, hipStream_t s,
+                       int n, const uint64_t *skeys, const int32_t *perm, const uint64_t *keys, int *child,
+                       int *index, int32_t *cell_level)
+{
+    lcp_kernel<<<capped_grid(n), 256, 0, s>>>(n, skeys, ts.lcp, st);
+    top_cells_kernel<<<(nsamples + 255) / 256, 256, 0, s>>>(g, nsamples, skeys, ts.top_flag, ts.top_id, ts.top_a,
+                                                             ts.top_b, st);
+    launch_exclusive_scan_u32(ts.top_id, nsamples * kLevels, s);
+    const int wire = nsamples * kLevels + g.nseg;
+    top_wire_kernel<<<(wire + 255) / 256, 256, 0, s>>>(g, nsamples, skeys, perm, child, cell_level, ts.top_flag,
+                                                        ts.top_id, ts.top_a, ts.top_b, ts.sub, ts.sub_max, base0, m,
+                                                        index, st);
+    subtree_build_kernel<<<kPersistentBlocks, 256, 0, s>>>(n, m, skeys, perm, keys, ts.lcp, child, index, cell_level,
+                                                           ts.sub, ts.subcells, st);
+}
+
+static void summarise_tree(Segs g, int nsamples, int base0, TreeScratch &ts, DeviceState *st, hipStream_t s,
+                           const SumOut &o)
+{
+    subtree_summary_kernel<<<kPersistentBlocks, 256, 0, s>>>(o, ts.sub, ts.subcells, st);
+    top_summary_kernel<<<1, 1024, 0, s>>>(o, g, nsamples, ts.top_flag, ts.top_id, base0, st);
+}
+
+static inline int samples_of(int n) { return (n + kSubtreeMax - 1) / kSubtreeMax; }
+
+bool alloc_tree_scratch(TreeScratch &ts, int n, int nseg, int samples)
+{
+    ts.samples_max = samples;
+    int64_t sub = 8 * ((int64_t)samples * kKeyLevels + nseg);
+    sub = std::min<int64_t>(sub, (int64_t)n / 2 + 8 * (int64_t)nseg + 8);
+    ts.sub_max = (int)sub;
+    const size_t tt = (size_t)samples * kLevels;
+    return hipMalloc((void **)&ts.lcp, (size_t)n + 1) == hipSuccess &&
+           hipMalloc((void **)&ts.top_flag, 4 * tt) == hipSuccess &&
+           hipMalloc((void **)&ts.top_id, 4 * tt) == hipSuccess &&
+           hipMalloc((void **)&ts.top_a, 4 * tt) == hipSuccess &&
+           hipMalloc((void **)&ts.top_b, 4 * tt) == hipSuccess &&
+           hipMalloc((void **)&ts.sub, sizeof(int4) * (size_t)ts.sub_max) == hipSuccess &&
+           hipMalloc((void **)&ts.subcells, sizeof(int2) * (size_t)ts.sub_max) == hipSuccess;
+}
+
+void free_tree_scratch(TreeScratch &ts)
+{
+    void *p[] = {ts.lcp, ts.top_flag, ts.top_id, ts.top_a, ts.top_b, ts.sub, ts.subcells};
+    for (void *q : p)
+        if (q) hipFree(q);
+    ts = TreeScratch{};
+}
+
 void launch_build(nbx_context *c, hipStream_t s, int n, int m, int *child, int *index)
 {
     if (n <= 0) return;
-    build_kernel<false><<<(n + 255) / 256, 256, 0, s>>>(n, m, c->skeys, c->perm, c->keys, child, index,
-                                                        c->cell_level, c->parent, c->arrive, c->st, nullptr, 0);
+    Segs g{nullptr, nullptr, 1, n, 0};
+    build_tree(g, samples_of(n), n, m, c->ts, c->st, s, n, c->skeys, c->perm, c->keys, child, index,
+               c->cell_level);
 }
 
 void launch_summarize(nbx_context *c, hipStream_t s)
 {
-    const int threads = c->m - c->n + 1;
-    summarize_kernel<<<(threads + 255) / 256, 256, 0, s>>>(
-        c->n, c->index, c->child, c->x, c->y, c->z, c->mass, c->x, c->y, c->z, c->count, c->start, c->pos,
-        c->parent, c->arrive, c->cell_level, c->mass64, c->mom64, c->nodes4, c->st);
+    Segs g{nullptr, nullptr, 1, c->n, 0};
+    SumOut o{c->n, c->child, c->x, c->y, c->z, c->mass, c->x, c->y, c->z, c->count, c->start, c->pos,
+             c->cell_level, c->mass64, c->mom64, c->nodes4};
+    summarise_tree(g, samples_of(c->n), c->n, c->ts, c->st, s, o);
 }
 
 void launch_layout(nbx_context *c, hipStream_t s, int *sorted)
@@ -329,17 +602,17 @@ void launch_layout(nbx_context *c, hipStream_t s, int *sorted)
 
 void launch_batch_build(nbx_batch *b, hipStream_t s)
 {
-    build_kernel<true><<<(b->N + 255) / 256, 256, 0, s>>>(b->N, b->m, b->skeys, b->perm, b->keys, b->child,
-                                                          b->index, b->cell_level, b->parent, b->arrive, b->st,
-                                                          b->sys_of, b->N);
+    Segs g{b->off, b->samp_off, b->S, b->N, 1};
+    build_tree(g, b->ts.samples_max, b->N + b->S, b->m, b->ts, b->st, s, b->N, b->skeys, b->perm, b->keys, b->child,
+               b->index, b->cell_level);
 }
 
 void launch_batch_summarize(nbx_batch *b, hipStream_t s)
 {
-    const int threads = b->m - b->N;
-    summarize_batch_kernel<<<(threads + 255) / 256, 256, 0, s>>>(
-        b->N, b->index, b->child, b->x, b->y, b->z, b->mass, b->count, b->start, b->pos, b->parent, b->arrive,
-        b->cell_level, b->mass64, b->mom64, b->nodes4, b->st);
+    Segs g{b->off, b->samp_off, b->S, b->N, 1};
+    SumOut o{b->N, b->child, b->x, b->y, b->z, b->mass, nullptr, nullptr, nullptr, b->count, b->start, b->pos,
+             b->cell_level, b->mass64, b->mom64, b->nodes4};
+    summarise_tree(g, b->ts.samples_max, b->N + b->S, b->ts, b->st, s, o);
 }
 
 void launch_batch_layout(nbx_batch *b, hipStream_t s)

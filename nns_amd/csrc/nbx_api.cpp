@@ -127,6 +127,7 @@ int nbx_create(const nbx_params *p, int n_max, int m_max, int device, nbx_contex
               dalloc(&c->acc_scratch, 3 * n) == hipSuccess &&
               hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
     for (int k = 0; ok && k < 8; ++k) ok = hipEventCreate(&c->stage_ev[k]) == hipSuccess;
+    if (ok) ok = nbx::alloc_tree_scratch(c->ts, n_max, 1, (n_max + nbx::kSubtreeMax - 1) / nbx::kSubtreeMax);
     if (ok) ok = hipMemset(c->st, 0, sizeof(DeviceState)) == hipSuccess;
     if (ok) ok = hipDeviceSynchronize() == hipSuccess;
     if (!ok) {
@@ -147,6 +148,7 @@ int nbx_destroy(nbx_context *c)
                     c->acc_scratch};
     for (void *b : bufs)
         if (b) hipFree(b);
+    nbx::free_tree_scratch(c->ts);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     for (auto &ev : c->stage_ev)

@@ -99,7 +99,7 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
     b->N = (int)total;
     b->m = (int)m64;
     b->force_waves = p->force_waves == 2 || p->force_waves == 8 ? p->force_waves : 4;
-    std::vector<int32_t> off(nsys + 1, 0), sys_of(total);
+    std::vector<int32_t> off(nsys + 1, 0), sys_of(total), samp(nsys + 1, 0);
     std::vector<float> th(nsys, p->theta);
     std::vector<int4> groups;
     for (int s = 0; s < nsys; ++s) {
@@ -107,6 +107,7 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
         for (int i = off[s]; i < off[s + 1]; ++i) sys_of[i] = s;
         if (sys_theta) th[s] = sys_theta[s];
         for (int g = 0; g < sys_n[s]; g += 64) groups.push_back(make_int4(off[s] + g, std::min(64, sys_n[s] - g), s, 0));
+        samp[s + 1] = samp[s] + (sys_n[s] + nbx::kSubtreeMax - 1) / nbx::kSubtreeMax;
     }
     b->ngroups = (int)groups.size();
     const size_t N = total, M = m64, S = nsys;
@@ -119,13 +120,15 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
               dalloc(&b->child, 8 * M) && dalloc(&b->index, 1) && dalloc(&b->count, M) && dalloc(&b->start, M) &&
               dalloc(&b->cell_level, M) && dalloc(&b->parent, M) && dalloc(&b->arrive, M) &&
               dalloc(&b->mass64, M) && dalloc(&b->mom64, 3 * M) && dalloc(&b->nodes4, M) &&
-              dalloc(&b->tchild, 8 * M) && dalloc(&b->counts, N) && dalloc(&b->acc_scratch, 3 * N);
+              dalloc(&b->tchild, 8 * M) && dalloc(&b->counts, N) && dalloc(&b->acc_scratch, 3 * N) &&
+              dalloc(&b->samp_off, S + 1) && nbx::alloc_tree_scratch(b->ts, (int)N, nsys, samp[nsys]);
     for (int k = 0; ok && k < 8; ++k) ok = hipEventCreate(&b->stage_ev[k]) == hipSuccess;
     ok = ok && hipMemset(b->st, 0, sizeof(nbx::DeviceState)) == hipSuccess &&
          hipMemcpy(b->off, off.data(), 4 * (S + 1), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(b->sys_of, sys_of.data(), 4 * N, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(b->theta, th.data(), 4 * S, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(b->groups, groups.data(), sizeof(int4) * groups.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(b->samp_off, samp.data(), 4 * (S + 1), hipMemcpyHostToDevice) == hipSuccess &&
          hipDeviceSynchronize() == hipSuccess;
     if (!ok) {
         nbx_batch_destroy(b);
@@ -144,9 +147,10 @@ int nbx_batch_destroy(nbx_batch *b)
     void *bufs[] = {b->st, b->off, b->sys_of, b->theta, b->groups, b->ext, b->mm, b->keys, b->skeys, b->kbuf,
                     b->k2, b->k2buf, b->perm, b->perm1, b->vbuf, b->pos, b->sorted, b->hist, b->child, b->index,
                     b->count, b->start, b->cell_level, b->parent, b->arrive, b->mass64, b->mom64, b->nodes4,
-                    b->tchild, b->counts, b->acc_scratch};
+                    b->tchild, b->counts, b->acc_scratch, b->samp_off};
     for (void *q : bufs)
         if (q) hipFree(q);
+    nbx::free_tree_scratch(b->ts);
     for (auto &ev : b->stage_ev)
         if (ev) hipEventDestroy(ev);
     delete b;

@@ -32,13 +32,38 @@ constexpr int kErrBucket = 2;
 constexpr int kErrStack = 4;
 constexpr int kErrNonFinite = 8;
 
+// Tree construction: cells holding more than kSubtreeMax bodies form the "top"
+// of the tree and are derived from the sorted keys; every child range of a top
+// cell with 2..kSubtreeMax bodies is a subtree built by one workgroup.
+constexpr int kSubtreeMax = 1024;
+constexpr int kLevels = kKeyLevels + 1;  // levels 0..21 per sample in the top table
+
 struct DeviceState {
     double ext[4];      // xMin, yMin, zMin, maxSpan
     double root64[4];   // root mass, com x, y, z
     float4 root4;       // root com x, y, z, mass (float)
     int err;            // latched kErr* bits
     unsigned bbox_ticket;
+    int nsub;           // subtrees found by the top-tree pass
+    int work;           // next subtree to build
+    int work2;          // next subtree to summarise
+    int ntop;           // top cells (excluding roots)
 };
+
+// Tree-construction scratch shared by the single-system and ensemble paths.
+struct TreeScratch {
+    uint8_t *lcp = nullptr;      // [n-1] common key digits of sorted neighbours
+    uint32_t *top_flag = nullptr, *top_id = nullptr;  // [samples * kLevels]
+    int32_t *top_a = nullptr, *top_b = nullptr;       // [samples * kLevels] body range of a top cell
+    int4 *sub = nullptr;         // {first, end, level, slot address}
+    int2 *subcells = nullptr;    // {first cell id, cells} of each subtree
+    int samples_max = 0, sub_max = 0;
+};
+
+// Allocates the scratch for up to n bodies in nseg segments with `samples` top-table
+// samples in total; returns false on allocation failure.
+bool alloc_tree_scratch(TreeScratch &ts, int n, int nseg, int samples);
+void free_tree_scratch(TreeScratch &ts);
 
 }  // namespace nbx
 
@@ -62,6 +87,7 @@ struct nbx_context {
     int32_t *tchild = nullptr;
     int64_t *counts = nullptr;
     float *acc_scratch = nullptr;
+    nbx::TreeScratch ts;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t stage_ev[8] = {};
 
@@ -102,6 +128,8 @@ struct nbx_batch {
     int32_t *tchild = nullptr;
     int64_t *counts = nullptr;
     float *acc_scratch = nullptr;
+    int32_t *samp_off = nullptr;  // [S+1] first top-table sample of each system
+    nbx::TreeScratch ts;
     hipEvent_t stage_ev[8] = {};
     // bound body arrays (caller-owned, N each)
     float *x = nullptr, *y = nullptr, *z = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
@@ -130,6 +158,7 @@ void launch_fill_f32(hipStream_t s, float *p, float v, int64_t n);
 // inv_out when not null.  tmp_k / tmp_v: scratch of n elements.
 void launch_radix_sort(nbx_context *c, hipStream_t s, const uint64_t *keys_in, int n, uint64_t *keys_out,
                        int32_t *perm_out, uint64_t *tmp_k, int32_t *tmp_v, int32_t *inv_out);
+void launch_exclusive_scan_u32(uint32_t *data, int total, hipStream_t s);
 // npasses passes of 8 bits from bit first_shift; vals_in == nullptr means 0, 1, 2, ...
 void launch_radix_passes(uint32_t *hist, hipStream_t s, const uint64_t *keys_in, const int32_t *vals_in, int n,
                          int first_shift, int npasses, uint64_t *keys_out, int32_t *vals_out, uint64_t *tmp_k,
