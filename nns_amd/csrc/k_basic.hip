@@ -132,15 +132,19 @@ __global__ void __launch_bounds__(256) bbox_kernel(
     }
 }
 
+// Keys, and (x, y, z, m) packed as float4 for the one-line gather of the layout.
 __global__ void __launch_bounds__(256) keys_kernel(
-    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z, int n,
-    const DeviceState *__restrict__ st, int key_type, uint64_t *__restrict__ keys)
+    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+    const float *__restrict__ mass, int n, const DeviceState *__restrict__ st, int key_type,
+    uint64_t *__restrict__ keys, float4 *__restrict__ bodies4)
 {
     const double ext[4] = {st->ext[0], st->ext[1], st->ext[2], st->ext[3]};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += blockDim.x * gridDim.x) {
-        uint64_t k = morton_key(x[i], y[i], z[i], ext);
+        const float xi = x[i], yi = y[i], zi = z[i];
+        uint64_t k = morton_key(xi, yi, zi, ext);
         if (key_type == NBX_KEY_HILBERT) k = hilbert_of(k & ~(1ull << 63));
         keys[i] = k;
+        if (mass) bodies4[i] = make_float4(xi, yi, zi, mass[i]);
     }
 }
 
@@ -195,11 +199,11 @@ void launch_bbox(nbx_context *c, hipStream_t s, const float *x, const float *y, 
     bbox_kernel<<<g, block, 0, s>>>(x, y, z, n, c->bbox_part, c->st, minX, maxX, minY, maxY, minZ, maxZ);
 }
 
-void launch_keys(nbx_context *c, hipStream_t s, const float *x, const float *y, const float *z, int n,
-                 int key_type, uint64_t *keys)
+void launch_keys(nbx_context *c, hipStream_t s, const float *x, const float *y, const float *z, const float *mass,
+                 int n, int key_type, uint64_t *keys)
 {
     const int block = 256;
-    keys_kernel<<<grid_for(n, block, 65536), block, 0, s>>>(x, y, z, n, c->st, key_type, keys);
+    keys_kernel<<<grid_for(n, block, 65536), block, 0, s>>>(x, y, z, mass, n, c->st, key_type, keys, c->bodies4);
 }
 
 void launch_update(hipStream_t s, int block, float *x, float *y, float *z, float *vx, float *vy, float *vz,
@@ -280,13 +284,16 @@ __global__ void __launch_bounds__(256) bbox_batch_kernel(
 }
 
 __global__ void __launch_bounds__(256) keys_batch_kernel(
-    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z, int n,
-    const int32_t *__restrict__ sys_of, const double *__restrict__ ext, uint64_t *__restrict__ keys)
+    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+    const float *__restrict__ mass, int n, const int32_t *__restrict__ sys_of, const double *__restrict__ ext,
+    uint64_t *__restrict__ keys, float4 *__restrict__ bodies4)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += blockDim.x * gridDim.x) {
         const int s = sys_of[i];
         const double e[4] = {ext[4 * s], ext[4 * s + 1], ext[4 * s + 2], ext[4 * s + 3]};
-        keys[i] = morton_key(x[i], y[i], z[i], e);
+        const float xi = x[i], yi = y[i], zi = z[i];
+        keys[i] = morton_key(xi, yi, zi, e);
+        bodies4[i] = make_float4(xi, yi, zi, mass[i]);
     }
 }
 
@@ -303,8 +310,8 @@ void launch_batch_bbox(nbx_batch *b, hipStream_t s)
 
 void launch_batch_keys(nbx_batch *b, hipStream_t s)
 {
-    keys_batch_kernel<<<grid_for(b->N, 256, 65536), 256, 0, s>>>(b->x, b->y, b->z, b->N, b->sys_of, b->ext,
-                                                                 b->keys);
+    keys_batch_kernel<<<grid_for(b->N, 256, 65536), 256, 0, s>>>(b->x, b->y, b->z, b->mass, b->N, b->sys_of,
+                                                                 b->ext, b->keys, b->bodies4);
 }
 
 }  // namespace nbx

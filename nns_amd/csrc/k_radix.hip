@@ -4,7 +4,7 @@
 // (4 waves x 16 items x 64 lanes; wave w owns the contiguous quarter w of the
 // tile, walked item by item so that lane order is element order):
 //   upsweep   per-tile digit histogram           reads 8 B/key
-//   scan      exclusive scan, digit-major          ~1 KiB per tile
+//   scan      exclusive scan of each digit row       ~1 KiB per tile
 //   downsweep stable rank + scatter                reads 12 B/key, writes 12 B/key
 // Ranks inside a wave come from a 64-lane match of the digit built from eight
 // ballots (no atomics, no intra-wave waits), so equal digits keep their order.
@@ -52,40 +52,101 @@ __global__ void __launch_bounds__(kSortThreads) radix_upsweep(
     }
 }
 
-// Exclusive scan of `total` counters in one block of 1024 threads.
-__global__ void __launch_bounds__(1024) radix_scan(uint32_t *__restrict__ hist, int total)
+// Block-wide exclusive scan of one value per thread (blockDim.x <= 1024); returns
+// the thread's exclusive prefix and writes the block total to *total.
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh, uint32_t *total)
 {
-    __shared__ uint32_t part[1024];
-    const int tid = threadIdx.x;
-    const int per = (total + 1023) / 1024;
-    const int lo = tid * per, hi = min(total, lo + per);
-    uint32_t s = 0;
-    for (int k = lo; k < hi; ++k) s += hist[k];
-    part[tid] = s;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    sh[tid] = v;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = tid >= off ? part[tid - off] : 0u;
+    for (int off = 1; off < nt; off <<= 1) {
+        const uint32_t u = tid >= off ? sh[tid - off] : 0u;
         __syncthreads();
-        part[tid] += v;
+        sh[tid] += u;
         __syncthreads();
     }
-    uint32_t run = part[tid] - s;
+    const uint32_t incl = sh[tid];
+    *total = sh[nt - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// Exclusive scan of every digit row hist[d][0..ntiles) in place (one block per
+// digit); row totals go to rowsum[d].  The digit bases (exclusive scan of the
+// row totals) are formed by each downsweep block.
+__global__ void __launch_bounds__(256) radix_rows_scan(uint32_t *__restrict__ hist, int ntiles,
+                                                       uint32_t *__restrict__ rowsum)
+{
+    __shared__ uint32_t sh[256];
+    uint32_t *row = hist + (int64_t)blockIdx.x * ntiles;
+    const int per = (ntiles + 255) / 256;
+    const int lo = threadIdx.x * per, hi = min(ntiles, lo + per);
+    uint32_t s = 0;
+    for (int k = lo; k < hi; ++k) s += row[k];
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(s, sh, &total);
     for (int k = lo; k < hi; ++k) {
-        uint32_t v = hist[k];
-        hist[k] = run;
+        const uint32_t v = row[k];
+        row[k] = run;
         run += v;
     }
+    if (threadIdx.x == 0) rowsum[blockIdx.x] = total;
+}
+
+// Generic exclusive scan of `total` uint32 (<= 1024 * 1024): per-block scans of
+// 1024 elements, a scan of the block sums, then the block offsets added.
+__global__ void __launch_bounds__(256) scan_blocks_kernel(uint32_t *__restrict__ data, int total,
+                                                          uint32_t *__restrict__ sums)
+{
+    __shared__ uint32_t sh[256];
+    const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+    uint32_t v[4], s = 0;
+    for (int j = 0; j < 4; ++j) {
+        v[j] = base + j < total ? data[base + j] : 0u;
+        s += v[j];
+    }
+    uint32_t blk;
+    uint32_t run = block_exclusive_scan(s, sh, &blk);
+    for (int j = 0; j < 4; ++j) {
+        if (base + j < total) data[base + j] = run;
+        run += v[j];
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = blk;
+}
+
+__global__ void __launch_bounds__(1024) scan_sums_kernel(uint32_t *__restrict__ sums, int nb)
+{
+    __shared__ uint32_t sh[1024];
+    const uint32_t v = threadIdx.x < nb ? sums[threadIdx.x] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan(v, sh, &tot);
+    if (threadIdx.x < nb) sums[threadIdx.x] = ex;
+}
+
+__global__ void __launch_bounds__(256) scan_add_kernel(uint32_t *__restrict__ data, int total,
+                                                       const uint32_t *__restrict__ sums)
+{
+    const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+    const uint32_t off = sums[blockIdx.x];
+    for (int j = 0; j < 4; ++j)
+        if (base + j < total) data[base + j] += off;
 }
 
 __global__ void __launch_bounds__(kSortThreads) radix_downsweep(
     const uint64_t *__restrict__ kin, const int32_t *__restrict__ vin, int n, int shift, int ntiles,
-    const uint32_t *__restrict__ hist, uint64_t *__restrict__ kout, int32_t *__restrict__ vout,
-    int32_t *__restrict__ inv)
+    const uint32_t *__restrict__ hist, const uint32_t *__restrict__ rowsum, uint64_t *__restrict__ kout,
+    int32_t *__restrict__ vout, int32_t *__restrict__ inv)
 {
     constexpr int W = kSortThreads / 64;
     __shared__ uint32_t run[W][kRadix];
+    __shared__ uint32_t dbase[kRadix];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int k = tid; k < W * kRadix; k += kSortThreads) (&run[0][0])[k] = 0u;
+    {   // digit bases: exclusive scan of the digit totals (kSortThreads == kRadix)
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan(rowsum[tid], dbase, &tot);
+        dbase[tid] = ex;
+    }
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kSortTile + (int64_t)wave * (kSortItems * 64);
     const uint64_t lt = lanemask_lt(lane);
@@ -110,7 +171,7 @@ __global__ void __launch_bounds__(kSortThreads) radix_downsweep(
     __syncthreads();
     // exclusive prefix over the waves of this tile, plus the tile's global offset
     for (int d = tid; d < kRadix; d += kSortThreads) {
-        uint32_t s = hist[(int64_t)d * ntiles + blockIdx.x];
+        uint32_t s = hist[(int64_t)d * ntiles + blockIdx.x] + dbase[d];
         for (int w = 0; w < W; ++w) {
             uint32_t c = run[w][d];
             run[w][d] = s;
@@ -145,9 +206,10 @@ void launch_radix_passes(uint32_t *hist, hipStream_t s, const uint64_t *keys_in,
         const bool to_out = ((npasses - 1 - p) % 2) == 0;
         uint64_t *ko = to_out ? keys_out : tmp_k;
         int32_t *vo = to_out ? vals_out : tmp_v;
+        uint32_t *rowsum = hist + (int64_t)ntiles * kRadix;  // kRadix totals after the rows
         radix_upsweep<<<ntiles, kSortThreads, 0, s>>>(ki, n, shift, ntiles, hist);
-        radix_scan<<<1, 1024, 0, s>>>(hist, ntiles * kRadix);
-        radix_downsweep<<<ntiles, kSortThreads, 0, s>>>(ki, vi, n, shift, ntiles, hist, ko, vo,
+        radix_rows_scan<<<kRadix, 256, 0, s>>>(hist, ntiles, rowsum);
+        radix_downsweep<<<ntiles, kSortThreads, 0, s>>>(ki, vi, n, shift, ntiles, hist, rowsum, ko, vo,
                                                         last ? inv_out : nullptr);
         ki = ko;
         vi = vo;
@@ -191,9 +253,13 @@ void launch_batch_sort(nbx_batch *b, hipStream_t s)
     gather_sorted_kernel<<<g, 256, 0, s>>>(n, b->perm, b->keys, b->skeys, b->pos);
 }
 
-void launch_exclusive_scan_u32(uint32_t *data, int total, hipStream_t s)
+void launch_exclusive_scan_u32(uint32_t *data, int total, uint32_t *sums, hipStream_t s)
 {
-    if (total > 0) radix_scan<<<1, 1024, 0, s>>>(data, total);
+    if (total <= 0) return;
+    const int nb = (total + 1023) / 1024;  // <= 1024 blocks (total <= 1M)
+    scan_blocks_kernel<<<nb, 256, 0, s>>>(data, total, sums);
+    scan_sums_kernel<<<1, 1024, 0, s>>>(sums, nb);
+    scan_add_kernel<<<nb, 256, 0, s>>>(data, total, sums);
 }
 
 void launch_radix_sort(nbx_context *c, hipStream_t s, const uint64_t *keys_in, int n, uint64_t *keys_out,

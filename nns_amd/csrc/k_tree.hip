@@ -88,6 +88,7 @@ __global__ void __launch_bounds__(256) lcp_kernel(int n, const uint64_t *__restr
         st->work = 0;
         st->work2 = 0;
         st->ntop = 0;
+        for (int L = 0; L < kLevels; ++L) st->topl[L] = 0;
     }
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n - 1; k += blockDim.x * gridDim.x) {
         const uint64_t x = (skeys[k] ^ skeys[k + 1]) & ~(1ull << 63);
@@ -137,7 +138,7 @@ __global__ void __launch_bounds__(256) top_wire_kernel(
     Segs g, int nsamples, const uint64_t *__restrict__ skeys, const int32_t *__restrict__ perm, int *child,
     int32_t *__restrict__ cell_level, const uint32_t *__restrict__ flag, const uint32_t *__restrict__ id,
     const int32_t *__restrict__ ta, const int32_t *__restrict__ tb, int4 *__restrict__ sub, int sub_max,
-    int base0, int m, int *index, DeviceState *st)
+    int32_t *__restrict__ top_list, int base0, int m, int *index, DeviceState *st)
 {
     if (latched(st)) return;
     const int total = nsamples * kLevels;
@@ -159,6 +160,7 @@ __global__ void __launch_bounds__(256) top_wire_kernel(
         s = seg_of_sample(g, k);
         if (c >= m) return;  // capacity error latched by thread 0
         cell_level[c] = L;
+        top_list[L * nsamples + atomicAdd(&st->topl[L], 1)] = c;
         if (L == kKeyLevels) {  // more than kSubtreeMax bodies on one key
             atomicOr(&st->err, kErrBucket);
             for (int d = 0; d < 8; ++d) child[8 * (int64_t)c + d] = -1;
@@ -211,6 +213,19 @@ __device__ inline int cas_slot_wg(int *p, int expect, int desired)
     return expect;
 }
 
+__device__ inline int block_max(int v, int *red)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    int s = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) s = max(s, red[w]);
+    __syncthreads();
+    return s;
+}
+
 __device__ inline int block_sum(int v, int *red)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -239,7 +254,7 @@ __device__ inline int block_sum(int v, int *red)
 __global__ void __launch_bounds__(256) subtree_build_kernel(
     int n, int m, const uint64_t *__restrict__ skeys, const int32_t *__restrict__ perm,
     const uint64_t *__restrict__ keys, const uint8_t *__restrict__ lcp, int *child, int *index,
-    int32_t *__restrict__ cell_level, int4 *__restrict__ sub, int2 *__restrict__ subcells, DeviceState *st)
+    int32_t *__restrict__ cell_level, int4 *__restrict__ sub, int4 *__restrict__ subcells, DeviceState *st)
 {
     __shared__ int s_job, s_base, s_next, s_bad;
     __shared__ int red[4];
@@ -255,10 +270,13 @@ __global__ void __launch_bounds__(256) subtree_build_kernel(
         const int4 jb = sub[job];
         const int a = jb.x, b = jb.y, L0 = jb.z, slot = jb.w;
         // exact cell count of the subtree from the common-prefix lengths
-        int contrib = 0;
-        for (int k = a + tid; k < b - 1; k += blockDim.x)
+        int contrib = 0, deep = L0;
+        for (int k = a + tid; k < b - 1; k += blockDim.x) {
             contrib += (k == a) ? (int)lcp[a] - L0 + 1 : max(0, (int)lcp[k] - (int)lcp[k - 1]);
+            deep = max(deep, (int)lcp[k]);
+        }
         const int ks = block_sum(contrib, red);
+        const int Lmax = block_max(deep, red);
         if (tid == 0) {
             const int base = atomicAdd(index, ks);
             s_bad = (int64_t)base + ks > (int64_t)m;
@@ -269,9 +287,9 @@ __global__ void __launch_bounds__(256) subtree_build_kernel(
                 for (int d = 0; d < 8; ++d) child[8 * (int64_t)base + d] = -1;
                 cell_level[base] = L0;
                 child[slot] = base;
-                subcells[job] = make_int2(base, ks);
+                subcells[job] = make_int4(base, ks, L0, Lmax);
             } else {
-                subcells[job] = make_int2(0, 0);
+                subcells[job] = make_int4(0, 0, L0, L0 - 1);
             }
         }
         __syncthreads();
@@ -440,8 +458,8 @@ __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool s
 // Subtree summary: persistent workgroups take subtrees from the queue; the
 // cells of a subtree are the contiguous ids [base, base + ks), summed deepest
 // level first with a barrier between levels.
-__global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const int4 *__restrict__ sub,
-                                                              const int2 *__restrict__ subcells, DeviceState *st)
+__global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const int4 *__restrict__ subcells,
+                                                              DeviceState *st)
 {
     __shared__ int s_job;
     if (latched(st)) return;
@@ -453,10 +471,9 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
         const int job = s_job;
         __syncthreads();
         if (job >= nsub) break;
-        const int2 sc = subcells[job];
-        const int L0 = sub[job].z;
-        const int base = sc.x, ks = sc.y;
-        for (int L = kKeyLevels; L >= L0; --L) {
+        const int4 sc = subcells[job];
+        const int base = sc.x, ks = sc.y, L0 = sc.z;
+        for (int L = sc.w; L >= L0; --L) {
             for (int c = base + tid; c < base + ks; c += blockDim.x)
                 if (o.cell_level[c] == L) summarise(o, c, st, false);
             __syncthreads();
@@ -466,14 +483,12 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
 
 // Top summary in one block: top cells deepest level first, then the roots.
 __global__ void __launch_bounds__(1024) top_summary_kernel(SumOut o, Segs g, int nsamples,
-                                                           const uint32_t *__restrict__ flag,
-                                                           const uint32_t *__restrict__ id, int base0,
-                                                           DeviceState *st)
+                                                           const int32_t *__restrict__ top_list, DeviceState *st)
 {
     if (latched(st)) return;
     for (int L = kKeyLevels; L >= 1; --L) {
-        for (int k = threadIdx.x; k < nsamples; k += blockDim.x)
-            if (flag[k * kLevels + L]) summarise(o, base0 + (int)id[k * kLevels + L], st, false);
+        const int cnt = st->topl[L];
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) summarise(o, top_list[L * nsamples + i], st, false);
         __syncthreads();
     }
     for (int s = threadIdx.x; s < g.nseg; s += blockDim.x) summarise(o, seg_root(g, s), st, !g.batch);
@@ -483,16 +498,16 @@ __global__ void __launch_bounds__(1024) top_summary_kernel(SumOut o, Segs g, int
 // Bodies in key order as float4 (x, y, z, m) at nodes4[0..n), and child
 // references with bodies renamed to their sorted positions (tchild[8c+s]);
 // cells keep their ids.  Also writes sorted[].
-__global__ void __launch_bounds__(256) layout_bodies_kernel(
-    int n, const int32_t *__restrict__ perm, const float *__restrict__ x, const float *__restrict__ y,
-    const float *__restrict__ z, const float *__restrict__ mass, float4 *__restrict__ nodes4,
-    int *__restrict__ sorted, const DeviceState *st)
+__global__ void __launch_bounds__(256) layout_bodies_kernel(int n, const int32_t *__restrict__ perm,
+                                                            const float4 *__restrict__ bodies4,
+                                                            float4 *__restrict__ nodes4, int *__restrict__ sorted,
+                                                            const DeviceState *st)
 {
     if (latched(st)) return;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += blockDim.x * gridDim.x) {
         const int b = perm[k];
         sorted[k] = b;
-        nodes4[k] = make_float4(x[b], y[b], z[b], mass[b]);
+        nodes4[k] = bodies4[b];
     }
 }
 
@@ -533,20 +548,20 @@ static void build_tree(Segs g, int nsamples, int base0, int m, TreeScratch &ts, 
     lcp_kernel<<<capped_grid(n), 256, 0, s>>>(n, skeys, ts.lcp, st);
     top_cells_kernel<<<(nsamples + 255) / 256, 256, 0, s>>>(g, nsamples, skeys, ts.top_flag, ts.top_id, ts.top_a,
                                                              ts.top_b, st);
-    launch_exclusive_scan_u32(ts.top_id, nsamples * kLevels, s);
+    launch_exclusive_scan_u32(ts.top_id, nsamples * kLevels, ts.scan_tmp, s);
     const int wire = nsamples * kLevels + g.nseg;
     top_wire_kernel<<<(wire + 255) / 256, 256, 0, s>>>(g, nsamples, skeys, perm, child, cell_level, ts.top_flag,
-                                                        ts.top_id, ts.top_a, ts.top_b, ts.sub, ts.sub_max, base0, m,
-                                                        index, st);
+                                                        ts.top_id, ts.top_a, ts.top_b, ts.sub, ts.sub_max, ts.top_list,
+                                                        base0, m, index, st);
     subtree_build_kernel<<<kPersistentBlocks, 256, 0, s>>>(n, m, skeys, perm, keys, ts.lcp, child, index, cell_level,
                                                            ts.sub, ts.subcells, st);
 }
 
-static void summarise_tree(Segs g, int nsamples, int base0, TreeScratch &ts, DeviceState *st, hipStream_t s,
+static void summarise_tree(Segs g, int nsamples, TreeScratch &ts, DeviceState *st, hipStream_t s,
                            const SumOut &o)
 {
-    subtree_summary_kernel<<<kPersistentBlocks, 256, 0, s>>>(o, ts.sub, ts.subcells, st);
-    top_summary_kernel<<<1, 1024, 0, s>>>(o, g, nsamples, ts.top_flag, ts.top_id, base0, st);
+    subtree_summary_kernel<<<kPersistentBlocks, 256, 0, s>>>(o, ts.subcells, st);
+    top_summary_kernel<<<1, 1024, 0, s>>>(o, g, nsamples, ts.top_list, st);
 }
 
 static inline int samples_of(int n) { return (n + kSubtreeMax - 1) / kSubtreeMax; }
@@ -564,12 +579,14 @@ bool alloc_tree_scratch(TreeScratch &ts, int n, int nseg, int samples)
            hipMalloc((void **)&ts.top_a, 4 * tt) == hipSuccess &&
            hipMalloc((void **)&ts.top_b, 4 * tt) == hipSuccess &&
            hipMalloc((void **)&ts.sub, sizeof(int4) * (size_t)ts.sub_max) == hipSuccess &&
-           hipMalloc((void **)&ts.subcells, sizeof(int2) * (size_t)ts.sub_max) == hipSuccess;
+           hipMalloc((void **)&ts.subcells, sizeof(int4) * (size_t)ts.sub_max) == hipSuccess &&
+           hipMalloc((void **)&ts.top_list, 4 * tt) == hipSuccess &&
+           hipMalloc((void **)&ts.scan_tmp, 4 * 1024) == hipSuccess && (int64_t)samples * kLevels <= (1 << 20);
 }
 
 void free_tree_scratch(TreeScratch &ts)
 {
-    void *p[] = {ts.lcp, ts.top_flag, ts.top_id, ts.top_a, ts.top_b, ts.sub, ts.subcells};
+    void *p[] = {ts.lcp, ts.top_flag, ts.top_id, ts.top_a, ts.top_b, ts.sub, ts.subcells, ts.scan_tmp, ts.top_list};
     for (void *q : p)
         if (q) hipFree(q);
     ts = TreeScratch{};
@@ -588,14 +605,13 @@ void launch_summarize(nbx_context *c, hipStream_t s)
     Segs g{nullptr, nullptr, 1, c->n, 0};
     SumOut o{c->n, c->child, c->x, c->y, c->z, c->mass, c->x, c->y, c->z, c->count, c->start, c->pos,
              c->cell_level, c->mass64, c->mom64, c->nodes4};
-    summarise_tree(g, samples_of(c->n), c->n, c->ts, c->st, s, o);
+    summarise_tree(g, samples_of(c->n), c->ts, c->st, s, o);
 }
 
 void launch_layout(nbx_context *c, hipStream_t s, int *sorted)
 {
     const int n = c->n;
-    layout_bodies_kernel<<<capped_grid(n), 256, 0, s>>>(n, c->perm, c->x, c->y, c->z, c->mass, c->nodes4, sorted,
-                                                        c->st);
+    layout_bodies_kernel<<<capped_grid(n), 256, 0, s>>>(n, c->perm, c->bodies4, c->nodes4, sorted, c->st);
     layout_child_kernel<false><<<capped_grid(8 * (int64_t)(c->m - c->n + 1)), 256, 0, s>>>(
         n, c->index, c->child, c->pos, c->tchild, c->st);
 }
@@ -612,13 +628,12 @@ void launch_batch_summarize(nbx_batch *b, hipStream_t s)
     Segs g{b->off, b->samp_off, b->S, b->N, 1};
     SumOut o{b->N, b->child, b->x, b->y, b->z, b->mass, nullptr, nullptr, nullptr, b->count, b->start, b->pos,
              b->cell_level, b->mass64, b->mom64, b->nodes4};
-    summarise_tree(g, b->ts.samples_max, b->N + b->S, b->ts, b->st, s, o);
+    summarise_tree(g, b->ts.samples_max, b->ts, b->st, s, o);
 }
 
 void launch_batch_layout(nbx_batch *b, hipStream_t s)
 {
-    layout_bodies_kernel<<<capped_grid(b->N), 256, 0, s>>>(b->N, b->perm, b->x, b->y, b->z, b->mass, b->nodes4,
-                                                           b->sorted, b->st);
+    layout_bodies_kernel<<<capped_grid(b->N), 256, 0, s>>>(b->N, b->perm, b->bodies4, b->nodes4, b->sorted, b->st);
     layout_child_kernel<true><<<capped_grid(8 * (int64_t)(b->m - b->N)), 256, 0, s>>>(
         b->N, b->index, b->child, b->pos, b->tchild, b->st);
 }

@@ -119,12 +119,12 @@ int nbx_create(const nbx_params *p, int n_max, int m_max, int device, nbx_contex
               dalloc(&c->keys, n) == hipSuccess && dalloc(&c->skeys, n) == hipSuccess &&
               dalloc(&c->kbuf, n) == hipSuccess && dalloc(&c->perm, n) == hipSuccess &&
               dalloc(&c->vbuf, n) == hipSuccess && dalloc(&c->pos, n) == hipSuccess &&
-              dalloc(&c->hist, (size_t)c->hist_tiles * nbx::kRadix) == hipSuccess &&
+              dalloc(&c->hist, (size_t)c->hist_tiles * nbx::kRadix + nbx::kRadix) == hipSuccess &&
               dalloc(&c->cell_level, m) == hipSuccess && dalloc(&c->parent, m) == hipSuccess &&
               dalloc(&c->arrive, m) == hipSuccess && dalloc(&c->mass64, m) == hipSuccess &&
               dalloc(&c->mom64, 3 * m) == hipSuccess && dalloc(&c->nodes4, m) == hipSuccess &&
               dalloc(&c->tchild, 8 * m) == hipSuccess && dalloc(&c->counts, n) == hipSuccess &&
-              dalloc(&c->acc_scratch, 3 * n) == hipSuccess &&
+              dalloc(&c->acc_scratch, 3 * n) == hipSuccess && dalloc(&c->bodies4, n) == hipSuccess &&
               hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
     for (int k = 0; ok && k < 8; ++k) ok = hipEventCreate(&c->stage_ev[k]) == hipSuccess;
     if (ok) ok = nbx::alloc_tree_scratch(c->ts, n_max, 1, (n_max + nbx::kSubtreeMax - 1) / nbx::kSubtreeMax);
@@ -145,7 +145,7 @@ int nbx_destroy(nbx_context *c)
     hipDeviceSynchronize();
     void *bufs[] = {c->st, c->bbox_part, c->keys, c->skeys, c->kbuf, c->perm, c->vbuf, c->pos, c->hist,
                     c->cell_level, c->parent, c->arrive, c->mass64, c->mom64, c->nodes4, c->tchild, c->counts,
-                    c->acc_scratch};
+                    c->acc_scratch, c->bodies4};
     for (void *b : bufs)
         if (b) hipFree(b);
     nbx::free_tree_scratch(c->ts);
@@ -246,7 +246,7 @@ int nbx_build_tree(nbx_context *c, float *x, float *y, float *z, float *mass, in
     c->child = child; c->count = count; c->start = start; c->index = index;
     c->n = n; c->m = m;
     StageTimer t(c, s, elapsed_ms);
-    nbx::launch_keys(c, s, x, y, z, n, NBX_KEY_MORTON, c->keys);
+    nbx::launch_keys(c, s, x, y, z, mass, n, NBX_KEY_MORTON, c->keys);
     nbx::launch_radix_sort(c, s, c->keys, n, c->skeys, c->perm, c->kbuf, c->vbuf, c->pos);
     nbx::launch_build(c, s, n, m, child, index);
     c->stage = 2;
@@ -327,7 +327,7 @@ int nbx_compute_keys(nbx_context *c, const float *x, const float *y, const float
     hipSetDevice(c->device);
     hipStream_t s = (hipStream_t)stream;
     StageTimer t(c, s, elapsed_ms);
-    nbx::launch_keys(c, s, x, y, z, n, key_type, keys);
+    nbx::launch_keys(c, s, x, y, z, nullptr, n, key_type, keys);
     return t.finish();
 }
 

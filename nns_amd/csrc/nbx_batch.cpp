@@ -116,11 +116,11 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
               dalloc(&b->groups, groups.size()) && dalloc(&b->ext, 4 * S) && dalloc(&b->mm, 6 * S) &&
               dalloc(&b->keys, N) && dalloc(&b->skeys, N) && dalloc(&b->kbuf, N) && dalloc(&b->k2, N) &&
               dalloc(&b->k2buf, N) && dalloc(&b->perm, N) && dalloc(&b->perm1, N) && dalloc(&b->vbuf, N) &&
-              dalloc(&b->pos, N) && dalloc(&b->sorted, N) && dalloc(&b->hist, tiles * nbx::kRadix) &&
+              dalloc(&b->pos, N) && dalloc(&b->sorted, N) && dalloc(&b->hist, tiles * nbx::kRadix + nbx::kRadix) &&
               dalloc(&b->child, 8 * M) && dalloc(&b->index, 1) && dalloc(&b->count, M) && dalloc(&b->start, M) &&
               dalloc(&b->cell_level, M) && dalloc(&b->parent, M) && dalloc(&b->arrive, M) &&
               dalloc(&b->mass64, M) && dalloc(&b->mom64, 3 * M) && dalloc(&b->nodes4, M) &&
-              dalloc(&b->tchild, 8 * M) && dalloc(&b->counts, N) && dalloc(&b->acc_scratch, 3 * N) &&
+              dalloc(&b->tchild, 8 * M) && dalloc(&b->counts, N) && dalloc(&b->acc_scratch, 3 * N) && dalloc(&b->bodies4, N) &&
               dalloc(&b->samp_off, S + 1) && nbx::alloc_tree_scratch(b->ts, (int)N, nsys, samp[nsys]);
     for (int k = 0; ok && k < 8; ++k) ok = hipEventCreate(&b->stage_ev[k]) == hipSuccess;
     ok = ok && hipMemset(b->st, 0, sizeof(nbx::DeviceState)) == hipSuccess &&
@@ -147,7 +147,7 @@ int nbx_batch_destroy(nbx_batch *b)
     void *bufs[] = {b->st, b->off, b->sys_of, b->theta, b->groups, b->ext, b->mm, b->keys, b->skeys, b->kbuf,
                     b->k2, b->k2buf, b->perm, b->perm1, b->vbuf, b->pos, b->sorted, b->hist, b->child, b->index,
                     b->count, b->start, b->cell_level, b->parent, b->arrive, b->mass64, b->mom64, b->nodes4,
-                    b->tchild, b->counts, b->acc_scratch, b->samp_off};
+                    b->tchild, b->counts, b->acc_scratch, b->samp_off, b->bodies4};
     for (void *q : bufs)
         if (q) hipFree(q);
     nbx::free_tree_scratch(b->ts);

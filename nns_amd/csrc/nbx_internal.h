@@ -48,6 +48,7 @@ struct DeviceState {
     int work;           // next subtree to build
     int work2;          // next subtree to summarise
     int ntop;           // top cells (excluding roots)
+    int topl[kLevels];  // top cells per level (lists in TreeScratch::top_list)
 };
 
 // Tree-construction scratch shared by the single-system and ensemble paths.
@@ -56,7 +57,9 @@ struct TreeScratch {
     uint32_t *top_flag = nullptr, *top_id = nullptr;  // [samples * kLevels]
     int32_t *top_a = nullptr, *top_b = nullptr;       // [samples * kLevels] body range of a top cell
     int4 *sub = nullptr;         // {first, end, level, slot address}
-    int2 *subcells = nullptr;    // {first cell id, cells} of each subtree
+    int4 *subcells = nullptr;    // {first cell id, cells, level, deepest level} of each subtree
+    int32_t *top_list = nullptr; // [kLevels * samples] top cells of each level
+    uint32_t *scan_tmp = nullptr;  // 1024 block sums of the top-table scan
     int samples_max = 0, sub_max = 0;
 };
 
@@ -87,6 +90,7 @@ struct nbx_context {
     int32_t *tchild = nullptr;
     int64_t *counts = nullptr;
     float *acc_scratch = nullptr;
+    float4 *bodies4 = nullptr;  // (x, y, z, m) of every body in input order
     nbx::TreeScratch ts;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t stage_ev[8] = {};
@@ -128,6 +132,7 @@ struct nbx_batch {
     int32_t *tchild = nullptr;
     int64_t *counts = nullptr;
     float *acc_scratch = nullptr;
+    float4 *bodies4 = nullptr;
     int32_t *samp_off = nullptr;  // [S+1] first top-table sample of each system
     nbx::TreeScratch ts;
     hipEvent_t stage_ev[8] = {};
@@ -147,8 +152,9 @@ void launch_reset(nbx_context *c, hipStream_t s, int *mutex, float *x, float *y,
                   float *minY, float *maxY, float *minZ, float *maxZ, int n, int m);
 void launch_bbox(nbx_context *c, hipStream_t s, const float *x, const float *y, const float *z,
                  float *minX, float *maxX, float *minY, float *maxY, float *minZ, float *maxZ, int n);
-void launch_keys(nbx_context *c, hipStream_t s, const float *x, const float *y, const float *z, int n,
-                 int key_type, uint64_t *keys);
+// Also packs (x, y, z, mass) into c->bodies4 when mass is given.
+void launch_keys(nbx_context *c, hipStream_t s, const float *x, const float *y, const float *z, const float *mass,
+                 int n, int key_type, uint64_t *keys);
 void launch_update(hipStream_t s, int block, float *x, float *y, float *z, float *vx, float *vy, float *vz,
                    const float *ax, const float *ay, const float *az, int n, float dt, float d);
 void launch_fill_f32(hipStream_t s, float *p, float v, int64_t n);
@@ -158,7 +164,8 @@ void launch_fill_f32(hipStream_t s, float *p, float v, int64_t n);
 // inv_out when not null.  tmp_k / tmp_v: scratch of n elements.
 void launch_radix_sort(nbx_context *c, hipStream_t s, const uint64_t *keys_in, int n, uint64_t *keys_out,
                        int32_t *perm_out, uint64_t *tmp_k, int32_t *tmp_v, int32_t *inv_out);
-void launch_exclusive_scan_u32(uint32_t *data, int total, hipStream_t s);
+// exclusive scan of total <= 2^20 values; sums: scratch of 1024 values
+void launch_exclusive_scan_u32(uint32_t *data, int total, uint32_t *sums, hipStream_t s);
 // npasses passes of 8 bits from bit first_shift; vals_in == nullptr means 0, 1, 2, ...
 void launch_radix_passes(uint32_t *hist, hipStream_t s, const uint64_t *keys_in, const int32_t *vals_in, int n,
                          int first_shift, int npasses, uint64_t *keys_out, int32_t *vals_out, uint64_t *tmp_k,
