@@ -7,7 +7,7 @@ softening 1e-3, G = 1, dt = 1e-3.  A step is the reference's full stage
 sequence (reset, bounding box, build, centre of mass, sort, forces, update) on
 arrays already resident in HBM.
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--theta T]
+    python bench.py [--steps K] [--warmup W] [--n N] [--theta T] [--skip ...]
 
 Prints one JSON line.  `value` is G interactions/s of the force stage, where
 interactions are the per-particle interaction lists (cells accepted by
@@ -15,6 +15,20 @@ l/D < theta plus leaf bodies) counted on the GPU outside the timed region.
 `roofline` prices the force kernel at 20 flop per interaction against the
 MI355X FP32 vector peak; `cpu_baseline` times the reference's own OpenMP tree
 code (oracle/_ref/ref_harness) on this box's host cores on a bounded sample.
+
+Beside the headline the line carries the other forms, each on its own
+workload:
+  plummer  -- Plummer sphere N = 1,048,576: median ms of 20 full steps;
+  batched  -- 512 Plummer spheres, N_s log-uniform in [1024, 65536], theta
+              cycling 0.3/0.5/0.7, one graph-captured launch per stage after the
+              autotuner; interactions/s against one Plummer sphere of the same
+              total N stepped by the plain path;
+  adjoint  -- Plummer N = 262,144, 64 steps of dt = 1e-3, loss = mean squared
+              distance of the final positions to seeded targets; the gradient
+              with respect to initial positions, velocities and masses through
+              the discrete adjoint (checkpoints every 8 steps); ratio of the
+              backward to the forward time, and central differences on 64
+              random coordinates of the initial positions.
 """
 from __future__ import annotations
 
@@ -44,11 +58,13 @@ def parse():
     ap.add_argument("--n", type=int, default=4194304)
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--accuracy-sample", type=int, default=2048,
+    ap.add_argument("--accuracy-sample", type=int, default=16384,
                     help="bodies checked against the FP64 direct sum (0 = skip)")
     ap.add_argument("--cpu-sample", type=int, default=131072,
                     help="bodies whose reference CPU tree walk is timed (0 = skip the CPU baseline)")
     ap.add_argument("--no-kahan", action="store_true")
+    ap.add_argument("--skip", default="", help="comma list of sections to skip: plummer,batched,adjoint")
+    ap.add_argument("--fd-coords", type=int, default=64, help="coordinates of the adjoint's central differences")
     return ap.parse_args()
 
 
@@ -81,6 +97,161 @@ def cpu_baseline(x, y, z, m, theta, softening, sample):
                        f"({rec['nsample']} bodies, {rec['force_sample_s']:.2f} s on {rec['threads']} threads), "
                        f"force scaled by n/sample"),
             "build_s": rec["build_s"], "force_s_extrapolated": round(force_full, 3)}
+
+
+def _median(v):
+    return float(np.median(np.asarray(v, np.float64)))
+
+
+def run_plummer(args):
+    """Plummer sphere N = 1,048,576, theta = 0.5: median of 20 timed steps."""
+    import torch
+    import nns_amd
+    n = 1048576
+    x, y, z, vx, vy, vz, m = nns_amd.initial_conditions("plummer", n, args.seed)
+    bh = nns_amd.BarnesHut(x, y, z, vx, vy, vz, m, params=nns_amd.default_params(theta=0.5))
+    for _ in range(3):
+        bh.update()
+    torch.cuda.synchronize()
+    st = [bh.update(stage_times=True) for _ in range(20)]
+    bh.ctx.check()
+    tot = [sum(t) for t in st]
+    counts = bh.count_interactions()
+    torch.cuda.synchronize()
+    i_alg = int(counts.sum().item())
+    force = _median([t[5] for t in st])
+    return {"workload": "Plummer sphere N=1048576 (M=1, a=1), theta=0.5, eps=1e-3, build+force+update",
+            "median_ms_per_step": round(_median(tot), 4), "min_ms_per_step": round(min(tot), 4),
+            "median_force_ms": round(force, 4), "interactions_per_step": i_alg,
+            "g_interactions_per_s_force": round(i_alg / force / 1e6, 2), "steps": 20}
+
+
+def run_batched(args, stream):
+    """512 Plummer spheres of uneven N against one sphere of the same total N."""
+    import torch
+    import nns_amd
+    rng = np.random.default_rng(args.seed + 7)
+    S = 512
+    sizes = np.exp(rng.uniform(np.log(1024.0), np.log(65536.0), S)).astype(np.int32)
+    thetas = np.array([0.3, 0.5, 0.7] * (S // 3 + 1), np.float32)[:S]
+    parts = [nns_amd.initial_conditions("plummer", int(k), seed=1000 + s) for s, k in enumerate(sizes)]
+    arrs = [np.concatenate([q[j] for q in parts]) for j in range(7)]
+    del parts
+    N = int(sizes.sum())
+    b = nns_amd.Batch(*arrs, sizes, thetas)
+    b.step()
+    b.check()
+    waves = b.autotune()
+    for _ in range(3):
+        b.graph_step(stream)
+    torch.cuda.synchronize()
+    c0 = int(b.count_interactions().sum().item())
+    K = 10
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(K):
+        b.graph_step(stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    b.check()
+    ms_b = e0.elapsed_time(e1) / K
+    c1 = int(b.count_interactions().sum().item())
+    stage_b = [b.step(stage_times=True) for _ in range(3)]
+    b.check()
+    i_b = 0.5 * (c0 + c1)
+    del b
+    torch.cuda.empty_cache()
+
+    x, y, z, vx, vy, vz, m = nns_amd.initial_conditions("plummer", N, args.seed + 11)
+    bh = nns_amd.BarnesHut(x, y, z, vx, vy, vz, m, params=nns_amd.default_params(theta=0.5))
+    for _ in range(3):
+        bh.update()
+    torch.cuda.synchronize()
+    st = [bh.update(stage_times=True) for _ in range(K)]
+    bh.ctx.check()
+    ms_p = _median([sum(t) for t in st])
+    i_p = int(bh.count_interactions().sum().item())
+    del bh
+    torch.cuda.empty_cache()
+    ips_b = i_b / ms_b / 1e6
+    ips_p = i_p / ms_p / 1e6
+    names = ("reset", "bbox", "build", "com", "sort", "force", "update")
+    return {"workload": f"512 Plummer spheres, N_s log-uniform in [1024, 65536] (total {N}), theta 0.3/0.5/0.7 "
+                        f"cycled; plain: one Plummer sphere N={N}, theta=0.5",
+            "systems": S, "total_n": N, "force_waves": waves,
+            "batched_ms_per_step": round(ms_b, 4), "batched_interactions_per_step": int(i_b),
+            "batched_g_interactions_per_s": round(ips_b, 2),
+            "batched_stage_ms": {k: round(_median([t[j] for t in stage_b]), 4) for j, k in enumerate(names)},
+            "plain_ms_per_step": round(ms_p, 4), "plain_interactions_per_step": i_p,
+            "plain_g_interactions_per_s": round(ips_p, 2),
+            "batched_over_plain": round(ips_b / ips_p, 3)}
+
+
+def run_adjoint(args):
+    """Gradient of a 64-step rollout (Plummer N = 262,144) and its cost."""
+    import torch
+    import nns_amd
+    from nns_amd.adjoint import Slots, rollout
+    n, steps, dt, every = 262144, 64, 1e-3, 8
+    x, y, z, vx, vy, vz, m = nns_amd.initial_conditions("plummer", n, args.seed)
+    tx, ty, tz = nns_amd.initial_conditions("plummer", n, args.seed + 1)[:3]
+    slots = Slots(n, every, nns_amd.default_params(theta=0.5))
+    dev = slots.bh[0].dev
+    tg = [torch.as_tensor(t, device=dev).double() for t in (tx, ty, tz)]
+
+    def loss_of(out):
+        return ((out[0].double() - tg[0]) ** 2 + (out[1].double() - tg[1]) ** 2
+                + (out[2].double() - tg[2]) ** 2).mean()
+
+    def fresh():
+        return [torch.tensor(a, device=dev, requires_grad=True) for a in (x, y, z, vx, vy, vz, m)]
+
+    w = fresh()  # warm-up: one segment forward and backward
+    loss_of(rollout(slots, *w, steps=every, dt=dt, every=every)).backward()
+    torch.cuda.synchronize()
+    ins = fresh()
+    t0 = time.perf_counter()
+    out = rollout(slots, *ins, steps=steps, dt=dt, every=every)
+    L = loss_of(out)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    L.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    slots.check()
+    # one step's pieces on slot 0: tree + forces, tree + VJP
+    bh = slots.bh[0]
+    u = [torch.ones(n, device=dev) for _ in range(3)]
+    fw = [sum(bh.rebuild(timing=True)) + bh.forces(timing=True) for _ in range(5)]
+    vj = [bh.vjp(*u, timing=True)[4] for _ in range(5)]
+    bh.ctx.check()
+
+    rng = np.random.default_rng(args.seed + 3)
+    picks = [(int(rng.integers(0, 3)), int(rng.integers(0, n))) for _ in range(args.fd_coords)]
+    h = 2e-3
+    rel = []
+    with torch.no_grad():
+        for ax_, i in picks:
+            vals = []
+            for sgn in (1.0, -1.0):
+                a = [t.detach().clone() for t in ins]
+                a[ax_][i] += sgn * h
+                vals.append(loss_of(rollout(slots, *a, steps=steps, dt=dt, every=every)).item())
+            fd = (vals[0] - vals[1]) / (2 * h)
+            ad = float(ins[ax_].grad[i].item())
+            rel.append(abs(fd - ad) / max(abs(ad), 1e-30))
+    rel = np.asarray(rel)
+    return {"workload": "Plummer sphere N=262144, theta=0.5, eps=1e-3, 64 steps of dt=1e-3, "
+                        "loss = mean |x_64 - target|^2, targets from a second seeded Plummer sphere",
+            "checkpoint_every": every, "loss": float(L.item()),
+            "forward_ms": round((t1 - t0) * 1e3, 2), "backward_ms": round((t2 - t1) * 1e3, 2),
+            "adjoint_over_forward": round((t2 - t1) / (t1 - t0), 3),
+            "tree_force_ms": round(_median(fw), 4), "vjp_ms": round(_median(vj), 4),
+            "grad_norms": {k: float(t.grad.norm().item()) for k, t in zip(("x", "y", "z", "vx", "vy", "vz", "m"),
+                                                                           ins)},
+            "fd_check": {"coords": len(picks), "h": h, "median_rel_err": float(np.median(rel)),
+                         "frac_within_1e-2": float((rel < 1e-2).mean()), "max_rel_err": float(rel.max())}}
 
 
 def main():
@@ -221,6 +392,13 @@ def main():
         "cpu_baseline": cpu,
         "accuracy": accuracy,
     }
+    skip = set(k for k in args.skip.split(",") if k)
+    if "plummer" not in skip:
+        out["plummer"] = run_plummer(args)
+    if "batched" not in skip:
+        out["batched"] = run_batched(args, stream)
+    if "adjoint" not in skip:
+        out["adjoint"] = run_adjoint(args)
     print(json.dumps(out))
 
 

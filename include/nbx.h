@@ -191,6 +191,18 @@ int nbx_copy_buffer(nbx_context *ctx, int which, void *host_dst, int64_t count, 
  * (nullable) receives the choice.  Synchronises the stream. */
 int nbx_autotune(nbx_context *ctx, void *stream, int *waves_out);
 
+/* ---- differentiation (no counterpart in the reference) ----------------------------- */
+
+/* Vector-Jacobian product of the accelerations over the tree of the last nbx_sort: for
+ * u = dL/da (device arrays of n floats, input order), writes dL/dx, dL/dy, dL/dz and
+ * dL/dm of every body (n floats each).  g is the gravity given to nbx_compute_forces.
+ * Each body meets exactly the nodes of its forward interaction list; cell gradients
+ * flow down to the bodies through the centre-of-mass sums.  Node sums use FP64
+ * atomics, so the last bits may differ between runs.  The first call allocates
+ * 64 (m_max + 1) bytes of scratch. */
+int nbx_force_vjp(nbx_context *ctx, const float *ux, const float *uy, const float *uz, float g, float *gx,
+                  float *gy, float *gz, float *gm, void *stream, float *elapsed_ms);
+
 /* ---- batched ensembles (no counterpart in the reference) --------------------------- */
 
 /* nsys independent systems in one set of concatenated arrays: system s owns bodies

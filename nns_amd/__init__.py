@@ -40,7 +40,7 @@ EXPORTS = [
     "nbx_sort_keys", "nbx_count_interactions", "nbx_step", "nbx_copy_buffer", "nbx_ic_star_disk",
     "nbx_ic_plummer", "nbx_ic_binary_disk", "nbx_autotune", "nbx_batch_create", "nbx_batch_destroy",
     "nbx_batch_last_error", "nbx_batch_bind", "nbx_batch_step", "nbx_batch_graph_step", "nbx_batch_check",
-    "nbx_batch_count_interactions", "nbx_batch_autotune", "nbx_batch_copy_buffer",
+    "nbx_batch_count_interactions", "nbx_batch_autotune", "nbx_batch_copy_buffer", "nbx_force_vjp",
 ]
 
 
@@ -93,6 +93,7 @@ def lib():
     L.nbx_compute_keys.argtypes = [vp, vp, vp, vp, i, i, vp, vp, fp]
     L.nbx_sort_keys.argtypes = [vp, vp, i, vp, vp, vp, fp]
     L.nbx_count_interactions.argtypes = [vp, i, vp, vp]
+    L.nbx_force_vjp.argtypes = [vp, vp, vp, vp, f, vp, vp, vp, vp, vp, fp]
     L.nbx_step.argtypes = [vp, C.POINTER(Arrays), vp, fp]
     L.nbx_copy_buffer.argtypes = [vp, i, vp, C.c_int64, vp]
     for name in ("nbx_ic_star_disk", "nbx_ic_plummer", "nbx_ic_binary_disk"):
@@ -321,6 +322,40 @@ class BarnesHut:
         out = torch.zeros(self.n, dtype=torch.int64, device=self.dev)
         self.ctx._rc(lib().nbx_count_interactions(self.ctx.h, self.n, out.data_ptr(), _stream(None)))
         return out
+
+    def rebuild(self, timing=False):
+        """Reset, bounding box, build, centre of mass and sort of the current
+        positions (the tree that nbx_compute_forces and nbx_force_vjp walk)."""
+        a, kw, n, m = self.arr, self.kw, self.n, self.m
+        return [kw.resetArrays(a["mutex"], a["x"], a["y"], a["z"], a["mass"], a["count"], a["start"], a["sorted"],
+                               a["child"], a["index"], a["minX"], a["maxX"], a["minY"], a["maxY"], a["minZ"],
+                               a["maxZ"], n, m, timing=timing),
+                kw.computeBoundingBox(a["mutex"], a["x"], a["y"], a["z"], a["minX"], a["maxX"], a["minY"],
+                                      a["maxY"], a["minZ"], a["maxZ"], n, timing=timing),
+                kw.buildTree(a["x"], a["y"], a["z"], a["mass"], a["count"], a["start"], a["child"], a["index"],
+                             a["minX"], a["maxX"], a["minY"], a["maxY"], a["minZ"], a["maxZ"], n, m,
+                             timing=timing),
+                kw.centreOfMass(a["x"], a["y"], a["z"], a["mass"], a["index"], n, timing=timing),
+                kw.sort(a["count"], a["start"], a["sorted"], a["child"], a["index"], n, timing=timing)]
+
+    def forces(self, timing=False):
+        a, p = self.arr, self.ctx.params
+        return self.kw.computeForces(a["x"], a["y"], a["z"], a["vx"], a["vy"], a["vz"], a["ax"], a["ay"], a["az"],
+                                     a["mass"], a["sorted"], a["child"], a["minX"], a["maxX"], self.n, p.gravity,
+                                     timing=timing)
+
+    def vjp(self, ux, uy, uz, g=None, timing=False):
+        """dL/dx, dL/dy, dL/dz, dL/dm for u = dL/da over the tree of the last sort
+        (device tensors of n floats; nbx_force_vjp)."""
+        import torch
+        out = [torch.empty(self.n, dtype=torch.float32, device=self.dev) for _ in range(4)]
+        us = [u.contiguous() for u in (ux, uy, uz)]
+        ms = C.c_float(0.0)
+        gv = self.ctx.params.gravity if g is None else g
+        self.ctx._rc(lib().nbx_force_vjp(self.ctx.h, *[u.data_ptr() for u in us], C.c_float(gv),
+                                         *[o.data_ptr() for o in out], _stream(None),
+                                         C.byref(ms) if timing else None))
+        return (*out, ms.value) if timing else tuple(out)
 
     def host(self, k, count=None):
         t = self.arr[k]

@@ -1,0 +1,130 @@
+"""Differentiable Barnes-Hut over the C-ABI.
+
+* ``BarnesHutAccel`` -- torch.autograd.Function: accelerations of bodies at
+  (x, y, z) with masses m; its backward is ``nbx_force_vjp`` on the tree of the
+  same positions (rebuilt if the context has walked another tree since).
+* ``Rollout`` -- K steps of the reference's integrator (v += dt a; x += d dt v,
+  3D/CUDA/CUDA_NBody/src/Kernels.cu:581-600, here with d = 1) as one autograd
+  node.  Its backward is the discrete adjoint of that map,
+      lv' = lv + dt lx,   lx <- lx + dt (dA/dx)^T lv',   lm <- lm + dt (dA/dm)^T lv',
+  taken step by step from the last state to the first.  States are
+  checkpointed every ``every`` steps; the backward re-runs one segment from its
+  checkpoint with one context per step, so that every step's tree is still
+  held when its vector-Jacobian product is taken (no second build).
+
+No counterpart in the reference; the force and update arithmetic is that of
+the plain method.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import BarnesHut, Params, default_params
+
+
+class Slots:
+    """`count` independent BarnesHut contexts for the same n (one per step of a
+    checkpoint segment)."""
+
+    def __init__(self, n: int, count: int, params: Params | None = None, device: int = 0):
+        self.n = int(n)
+        self.params = params if params is not None else default_params()
+        z = np.zeros(self.n, np.float32)
+        self.bh = [BarnesHut(z, z, z, z, z, z, z, params=self.params, device=device) for _ in range(count)]
+
+    def accel(self, k: int, x, y, z, m):
+        """Builds slot k's tree for (x, y, z, m) and returns its accelerations (new tensors)."""
+        bh, n = self.bh[k], self.n
+        for key, t in (("x", x), ("y", y), ("z", z), ("mass", m)):
+            bh.arr[key][:n].copy_(t)
+        bh.rebuild()
+        bh.forces()
+        return bh.arr["ax"][:n].clone(), bh.arr["ay"][:n].clone(), bh.arr["az"][:n].clone()
+
+    def vjp(self, k: int, ux, uy, uz):
+        """(dL/dx, dL/dy, dL/dz, dL/dm) over slot k's current tree."""
+        return self.bh[k].vjp(ux, uy, uz, g=self.params.gravity)
+
+    def check(self):
+        for bh in self.bh:
+            bh.ctx.check()
+
+
+class BarnesHutAccel(torch.autograd.Function):
+    """a(x, y, z, m) through slot 0 of `slots`; backward through nbx_force_vjp."""
+
+    @staticmethod
+    def forward(ctx, x, y, z, m, slots: Slots):
+        ax, ay, az = slots.accel(0, x, y, z, m)
+        ctx.slots = slots
+        ctx.save_for_backward(x, y, z, m)
+        return ax, ay, az
+
+    @staticmethod
+    def backward(ctx, gax, gay, gaz):
+        x, y, z, m = ctx.saved_tensors
+        slots = ctx.slots
+        zero = torch.zeros_like(x)
+        u = [g.contiguous() if g is not None else zero for g in (gax, gay, gaz)]
+        slots.accel(0, x, y, z, m)  # the tree of these positions
+        gx, gy, gz, gm = slots.vjp(0, *u)
+        return gx, gy, gz, gm, None
+
+
+def _step(slots: Slots, k: int, s, m, dt: float):
+    x, y, z, vx, vy, vz = s
+    ax, ay, az = slots.accel(k, x, y, z, m)
+    vx = vx + dt * ax
+    vy = vy + dt * ay
+    vz = vz + dt * az
+    return (x + dt * vx, y + dt * vy, z + dt * vz, vx, vy, vz)
+
+
+class Rollout(torch.autograd.Function):
+    """Final state (x, y, z, vx, vy, vz) after `steps` steps from the initial
+    state and masses; gradients w.r.t. all seven inputs."""
+
+    @staticmethod
+    def forward(ctx, x, y, z, vx, vy, vz, m, slots: Slots, steps: int, dt: float, every: int):
+        s = (x, y, z, vx, vy, vz)
+        ckpt = []
+        for k in range(steps):
+            if k % every == 0:
+                ckpt.append(tuple(t.clone() for t in s))
+            s = _step(slots, k % every, s, m, dt)
+        ctx.slots, ctx.steps, ctx.dt, ctx.every = slots, steps, dt, every
+        ctx.ckpt = ckpt
+        ctx.save_for_backward(m)
+        return tuple(t.clone() for t in s)
+
+    @staticmethod
+    def backward(ctx, gx, gy, gz, gvx, gvy, gvz):
+        (m,) = ctx.saved_tensors
+        slots, steps, dt, every = ctx.slots, ctx.steps, ctx.dt, ctx.every
+        zero = torch.zeros_like(m)
+        lx, ly, lz, lvx, lvy, lvz = [g if g is not None else zero for g in (gx, gy, gz, gvx, gvy, gvz)]
+        lm = torch.zeros_like(m)
+        nseg = (steps + every - 1) // every
+        for seg in reversed(range(nseg)):
+            k0 = seg * every
+            k1 = min(steps, k0 + every)
+            # re-run the segment: slot j keeps the tree of state k0 + j
+            s = ctx.ckpt[seg]
+            for k in range(k0, k1):
+                s = _step(slots, k - k0, s, m, dt)
+            for k in reversed(range(k0, k1)):
+                # x_{k+1} = x_k + dt v_{k+1}, v_{k+1} = v_k + dt a(x_k)
+                lvx = lvx + dt * lx
+                lvy = lvy + dt * ly
+                lvz = lvz + dt * lz
+                ax, ay, az, am = slots.vjp(k - k0, lvx, lvy, lvz)
+                lx = lx + dt * ax
+                ly = ly + dt * ay
+                lz = lz + dt * az
+                lm = lm + dt * am
+        return lx, ly, lz, lvx, lvy, lvz, lm, None, None, None, None
+
+
+def rollout(slots: Slots, x, y, z, vx, vy, vz, m, steps: int, dt: float, every: int = 8):
+    return Rollout.apply(x, y, z, vx, vy, vz, m, slots, steps, dt, every)

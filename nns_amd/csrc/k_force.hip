@@ -1,40 +1,46 @@
 // k_force.hip -- Barnes-Hut accelerations: 64-lane wavefront traversal.
 //
 // One wave owns a group of up to 64 consecutive bodies in key order, one body
-// per lane.  The wave walks the tree depth first with an LDS stack of entries
-// {node, level, lane mask}.  The node of an iteration is the same for every
-// lane: it is read once from LDS, made scalar with readfirstlane, and its
-// float4 comes in through the scalar cache (s_load_dwordx4).  Each lane applies
-// its own opening test to a cell (l/D < theta written as crit_L < |d|^2 with
-// crit_L = (l_L/theta)^2; the reference CPU test of
+// per lane, and walks the tree depth first.  The unit of the walk is a frame:
+// an opened cell together with the mask of the lanes that opened it.  A frame
+// names a block of the frame layout (the cell's non-empty children, compacted,
+// with chains of one-child cells collapsed to their bottom cell), and the
+// whole block -- 8 float4 (x, y, z, m) and 8 int2 descriptors, 192 bytes --
+// comes in through the scalar cache with three s_load_dwordx16 once per frame.
+// The children are then stepped through with the data already in SGPRs, so
+// the only dependent memory access of a frame is that one load; frames live in
+// an LDS stack of {block, nch | level << 4, lane mask}.
+//
+// Each lane applies its own opening test to a cell child (l/D < theta written
+// as crit_L < |d|^2 with crit_L = (l_L/theta)^2; the reference CPU test of
 // 3D/OpenMP/OpenMP_NBody/src/Interaction.cpp:64-67).  The test is a ballot, so
-// the lane masks stay in SGPRs: lanes that accept add the cell's monopole under
-// exec = mask & accept, lanes that reject vote the cell open and its children
-// are pushed with mask & ~accept.  Every lane therefore sees exactly its own
-// per-particle interaction list; the wave pays for the union of the 64 lists.
-// Leaves (bodies) are always direct (Interaction.cpp:60-63); a zero distance
-// contributes nothing (:28-30).
+// lane masks stay in SGPRs: lanes that accept add the cell's monopole under
+// exec = mask & accept, lanes that reject push the child's frame with
+// mask & ~accept.  Every lane therefore sees exactly its own per-particle
+// interaction list; the wave pays for the union of the 64 lists.  Bodies are
+// always direct (Interaction.cpp:60-63); a zero distance contributes nothing
+// (:28-30).
 //
 // Interaction: a += m d (|d|^2 + eps^2)^-3/2, d = q - x (the 20-flop form of
 // GPU Gems 3 ch. 31), with the raw v_rsq_f32 (|d|^2 + eps^2 is never denormal
 // when eps > 0; the unsoftened variant masks d = 0).  With Kahan on, monopoles
 // go into a plain partial that is folded into a compensated total every 16
-// nodes.
+// children.
 //
 // Blocks are remapped so that the blocks one XCD receives (dispatch is
 // round-robin over the 8 XCDs) walk consecutive groups and share its L2.
 //
-// Single system: group g is sorted positions [64g, 64g+64), the root is the
-// special entry kRootRef (data in DeviceState, slots at tchild[0..7]).
-// Ensemble: group g comes from a table {first, count, system}; the walk starts
-// at the system's root cell with the system's own theta and extent.
+// Single system: group g is sorted positions [64g, 64g+64) and the walk starts
+// from froot[0].  Ensemble: group g comes from a table {first, count, system};
+// the walk starts at froot[system] with the system's own theta and extent.
 #include "nbx_device.h"
 
 namespace nbx {
 
 template <bool KAHAN, bool COUNT, bool SOFT, bool BATCH, int W>
 __global__ void __launch_bounds__(64 * W) force_kernel(
-    int n, int ngroups, const float4 *__restrict__ nodes4, const int32_t *__restrict__ tchild,
+    int n, int ngroups, const float4 *__restrict__ nodes4, const float4 *__restrict__ fnode,
+    const int2 *__restrict__ fmeta, const float4 *__restrict__ froot4, const int2 *__restrict__ frootm,
     const int32_t *__restrict__ perm, const DeviceState *st, float theta, float eps2, float G,
     float *__restrict__ ax, float *__restrict__ ay, float *__restrict__ az, int64_t *__restrict__ counts,
     int *__restrict__ err, const int4 *__restrict__ groups, const float *__restrict__ sys_theta,
@@ -47,20 +53,20 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
     const int g = xcd_swizzle(blockIdx.x, gridDim.x) * W + wave;
     if (g >= ngroups) return;
 
-    int first, cnt_bodies, root_ref;
+    int first, cnt_bodies, sys;
     double span;
     float th;
     if (BATCH) {
         const int4 gr = groups[g];
         first = gr.x;
         cnt_bodies = gr.y;
-        root_ref = n + gr.z;
-        span = sys_ext[4 * gr.z + 3];
-        th = sys_theta[gr.z];
+        sys = gr.z;
+        span = sys_ext[4 * sys + 3];
+        th = sys_theta[sys];
     } else {
         first = g * 64;
         cnt_bodies = min(64, n - first);
-        root_ref = kRootRef;
+        sys = 0;
         span = st->ext[3];
         th = theta;
     }
@@ -68,8 +74,6 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
     const double lc0 = ldexp(span / kMaskCoord, kKeyLevels);
     const double r0 = lc0 / (double)th;
     const float crit0 = (float)(r0 * r0);
-    float4 root = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (!BATCH) root = st->root4;
 
     const int p = first + lane;
     const bool valid = lane < cnt_bodies;
@@ -82,35 +86,35 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
     int64_t cnt = 0;
     int tile = 0;
 
-    uint4 *s = stk[wave];
-    if (lane == 0) s[0] = make_uint4((unsigned)root_ref, 0u, (unsigned)live, (unsigned)(live >> 32));
-    int sp = 1;
-    while (sp > 0) {
-        --sp;
-        const uint4 e = s[sp];
-        const int ref = __builtin_amdgcn_readfirstlane((int)e.x);
-        const int lev = __builtin_amdgcn_readfirstlane((int)e.y);
-        const uint64_t mask = ((uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)e.w) << 32) |
-                              (uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)e.z);
-        const bool is_root = !BATCH && ref == kRootRef;
-        float4 q = nodes4[is_root ? 0 : ref];  // index kept in bounds for the root entry
-        if (is_root) q = root;
+    // crit_L = crit_0 4^-L: subtracting 2L from the exponent field is exact
+    // while the result stays normal; otherwise ldexpf.
+    const int cb0 = __float_as_int(crit0);
+    const int e0 = (cb0 >> 23) & 0xff;
+    auto crit_at = [&](const int lev) -> float {
+        int b;
+        if (e0 > 2 * lev && e0 < 255) b = cb0 - (lev << 24);
+        else b = __builtin_amdgcn_readfirstlane(__float_as_int(ldexpf(crit0, -2 * lev)));
+        return __int_as_float(b);
+    };
+
+    // One child step: every lane of `mask` meets child q with descriptor
+    // (dx_, dy_); returns the lanes that open it.
+    auto step = [&](const float4 q, const int dx_, const int dy_, const uint64_t mask) -> uint64_t {
         const float dx = q.x - me.x;
         const float dy = q.y - me.y;
         const float dz = q.z - me.z;
         const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         uint64_t takem, omask;
-        if (ref < n) {
+        if (dy_ == 0) {
             takem = mask;
             omask = 0;
         } else {
-            const float crit = ldexpf(crit0, -2 * lev);
-            const uint64_t ok = __ballot(crit < d2);
+            const uint64_t ok = __ballot(crit_at(dy_ >> 4) < d2);
             takem = mask & ok;
             omask = mask & ~ok;
         }
         if (COUNT) {
-            if (__builtin_amdgcn_inverse_ballot_w64(takem)) cnt += (ref == p) ? 0 : 1;
+            if (__builtin_amdgcn_inverse_ballot_w64(takem)) cnt += (dy_ == 0 && dx_ == p) ? 0 : 1;
         }
         if (!SOFT) takem &= __ballot(d2 > 0.f);
         if (__builtin_amdgcn_inverse_ballot_w64(takem)) {
@@ -122,28 +126,71 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
             pz = fmaf(f, dz, pz);
         }
         if (KAHAN) {
-            if (++tile == 16) {
+            tile = __builtin_amdgcn_readfirstlane(tile + 1);
+            if (tile == 16) {
                 kahan_add(sx, cx, px); kahan_add(sy, cy, py); kahan_add(sz, cz, pz);
                 px = py = pz = 0.f;
                 tile = 0;
             }
         }
-        if (omask) {
-            const int base = is_root ? 0 : 8 * ref;
-            const int ch = lane < 8 ? tchild[base + lane] : -1;
-            const uint64_t vm = __ballot(ch != -1);
-            const int nch = __popcll(vm);
-            if (sp + nch > kForceStack) {
-                if (lane == 0) atomicOr(err, kErrStack);
-                break;
+        return omask;
+    };
+
+    uint4 *s = stk[wave];
+    int sp = 0;
+    {
+        const int2 rd = frootm[sys];
+        const int rx = __builtin_amdgcn_readfirstlane(rd.x), ry = __builtin_amdgcn_readfirstlane(rd.y);
+        const uint64_t om = step(froot4[sys], rx, ry, live);
+        if (om) {
+            if (lane == 0) s[0] = make_uint4((unsigned)rx, (unsigned)ry, (unsigned)om, (unsigned)(om >> 32));
+            sp = 1;
+        }
+    }
+    bool overflow = false;
+    while (sp > 0) {
+        sp = __builtin_amdgcn_readfirstlane(sp - 1);
+        const uint4 e = s[sp];
+        const int blk = __builtin_amdgcn_readfirstlane((int)e.x);
+        const int nch = __builtin_amdgcn_readfirstlane((int)e.y) & 15;
+        const uint64_t mask = ((uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)e.w) << 32) |
+                              (uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)e.z);
+        const float4 *bq = fnode + 8 * (int64_t)blk;
+        const int2 *bd = fmeta + 8 * (int64_t)blk;
+        float4 q[8];
+        int2 d[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            q[j] = bq[j];
+            d[j] = bd[j];
+        }
+        // Every load of the block is issued before the first use: one wait per frame.
+        asm volatile("" ::"s"(q[0].x), "s"(q[0].y), "s"(q[0].z), "s"(q[0].w), "s"(q[1].x), "s"(q[1].y),
+                     "s"(q[1].z), "s"(q[1].w), "s"(q[2].x), "s"(q[2].y), "s"(q[2].z), "s"(q[2].w), "s"(q[3].x),
+                     "s"(q[3].y), "s"(q[3].z), "s"(q[3].w));
+        asm volatile("" ::"s"(q[4].x), "s"(q[4].y), "s"(q[4].z), "s"(q[4].w), "s"(q[5].x), "s"(q[5].y),
+                     "s"(q[5].z), "s"(q[5].w), "s"(q[6].x), "s"(q[6].y), "s"(q[6].z), "s"(q[6].w), "s"(q[7].x),
+                     "s"(q[7].y), "s"(q[7].z), "s"(q[7].w));
+        asm volatile("" ::"s"(d[0].x), "s"(d[0].y), "s"(d[1].x), "s"(d[1].y), "s"(d[2].x), "s"(d[2].y),
+                     "s"(d[3].x), "s"(d[3].y), "s"(d[4].x), "s"(d[4].y), "s"(d[5].x), "s"(d[5].y), "s"(d[6].x),
+                     "s"(d[6].y), "s"(d[7].x), "s"(d[7].y));
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (j >= nch) break;
+            const int cx_ = __builtin_amdgcn_readfirstlane(d[j].x), cy_ = __builtin_amdgcn_readfirstlane(d[j].y);
+            const uint64_t om = step(q[j], cx_, cy_, mask);
+            if (om) {
+                if (sp >= kForceStack) {
+                    overflow = true;
+                    break;
+                }
+                if (lane == 0) s[sp] = make_uint4((unsigned)cx_, (unsigned)cy_, (unsigned)om, (unsigned)(om >> 32));
+                sp = __builtin_amdgcn_readfirstlane(sp + 1);
             }
-            if (ch != -1) {
-                // slot 0 ends on top of the stack, so children are visited in slot order
-                const int above = __popcll(vm & ~((2ull << lane) - 1ull));
-                s[sp + above] = make_uint4((unsigned)ch, (unsigned)(lev + 1), (unsigned)omask,
-                                           (unsigned)(omask >> 32));
-            }
-            sp += nch;
+        }
+        if (overflow) {
+            if (lane == 0) atomicOr(err, kErrStack);
+            break;
         }
     }
     if (KAHAN) {
@@ -162,8 +209,9 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
 
 struct ForceArgs {
     int n, ngroups;
-    const float4 *nodes4;
-    const int32_t *tchild, *perm;
+    const float4 *nodes4, *fnode, *froot4;
+    const int2 *fmeta, *frootm;
+    const int32_t *perm;
     const DeviceState *st;
     float theta, eps2, G;
     float *ax, *ay, *az;
@@ -179,7 +227,7 @@ static void launch_one(const ForceArgs &a, hipStream_t s)
 {
     const int nblk = (a.ngroups + W - 1) / W;
     force_kernel<KAHAN, COUNT, SOFT, BATCH, W><<<nblk, 64 * W, 0, s>>>(
-        a.n, a.ngroups, a.nodes4, a.tchild, a.perm, a.st, a.theta, a.eps2, a.G, a.ax, a.ay, a.az, a.counts, a.err,
+        a.n, a.ngroups, a.nodes4, a.fnode, a.fmeta, a.froot4, a.frootm, a.perm, a.st, a.theta, a.eps2, a.G, a.ax, a.ay, a.az, a.counts, a.err,
         a.groups, a.sys_theta, a.sys_ext);
 }
 
@@ -212,7 +260,8 @@ static ForceArgs single_args(nbx_context *c, float *ax, float *ay, float *az, fl
     ForceArgs a{};
     a.n = c->n;
     a.ngroups = (c->n + 63) / 64;
-    a.nodes4 = c->nodes4; a.tchild = c->tchild; a.perm = c->perm; a.st = c->st;
+    a.nodes4 = c->nodes4; a.fnode = c->fnode; a.fmeta = c->fmeta; a.froot4 = c->froot4; a.frootm = c->frootm;
+    a.perm = c->perm; a.st = c->st;
     a.theta = c->p.theta;
     a.eps2 = c->p.softening * c->p.softening;
     a.G = g;
@@ -238,7 +287,8 @@ void launch_batch_force(nbx_batch *b, hipStream_t s, int64_t *counts, int waves)
     ForceArgs a{};
     a.n = b->N;
     a.ngroups = b->ngroups;
-    a.nodes4 = b->nodes4; a.tchild = b->tchild; a.perm = b->perm; a.st = b->st;
+    a.nodes4 = b->nodes4; a.fnode = b->fnode; a.fmeta = b->fmeta; a.froot4 = b->froot4; a.frootm = b->frootm;
+    a.perm = b->perm; a.st = b->st;
     a.theta = b->p.theta;
     a.eps2 = b->p.softening * b->p.softening;
     a.G = b->p.gravity;

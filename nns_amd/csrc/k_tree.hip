@@ -495,9 +495,8 @@ __global__ void __launch_bounds__(1024) top_summary_kernel(SumOut o, Segs g, int
 }
 
 // ------------------------------------------------------------------ layout
-// Bodies in key order as float4 (x, y, z, m) at nodes4[0..n), and child
-// references with bodies renamed to their sorted positions (tchild[8c+s]);
-// cells keep their ids.  Also writes sorted[].
+// Bodies in key order as float4 (x, y, z, m) at nodes4[0..n) and sorted[];
+// then the frame blocks of the force walk (frame_layout_kernel).
 __global__ void __launch_bounds__(256) layout_bodies_kernel(int n, const int32_t *__restrict__ perm,
                                                             const float4 *__restrict__ bodies4,
                                                             float4 *__restrict__ nodes4, int *__restrict__ sorted,
@@ -511,23 +510,88 @@ __global__ void __launch_bounds__(256) layout_bodies_kernel(int n, const int32_t
     }
 }
 
-// Single system: item 0 is the root (child[0..7]), item i >= 1 is cell n + i - 1.
-// Ensemble: item i is cell n + i (root cells included).
+// Frame blocks.  Block b lists the non-empty children of one cell, compacted in
+// slot order: fnode[8b + k] = (x, y, z, m) of the child and fmeta[8b + k] its
+// descriptor, {sorted position, 0} for a body and {block, nch | level << 4}
+// for a cell.  A cell child is described by the bottom of its chain, the first
+// descendant with two or more children: the cells of a one-child chain carry
+// the same mass and centre of mass (the FP64 sums of a single child are exact
+// copies), and crit_L = crit_0 4^-L falls with L, so a body accepts some cell
+// of the chain exactly when it accepts the bottom one, and meets the same
+// monopole once either way.  fnode keeps the data of the child itself.
+// Single system: block 0 is the root (slots child[0..7]), block i >= 1 is cell
+// n + i - 1.  Ensemble: block i is cell n + i, the roots being blocks 0..S-1.
+// froot4 / frootm describe each root as a child of a virtual parent.  The
+// parent blocks of every block and of every body (by sorted position) go to
+// pblk / bpar for the adjoint's top-down pass.
 template <bool BATCH>
-__global__ void __launch_bounds__(256) layout_child_kernel(
-    int n, const int *__restrict__ index, const int *__restrict__ child, const int32_t *__restrict__ pos,
-    int32_t *__restrict__ tchild, const DeviceState *st)
+__device__ inline int block_of(int c, int n)
+{
+    return BATCH ? c - n : (c == 0 ? 0 : c - n + 1);
+}
+
+// Bottom of the one-child chain starting at cell c (c = 0 is the single-system
+// root); returns {block, nch | level << 4}.
+template <bool BATCH>
+__device__ inline int2 chain_bottom(int c, int n, const int *__restrict__ child,
+                                    const int32_t *__restrict__ cell_level, bool root)
+{
+    for (;;) {
+        const int4 *p = reinterpret_cast<const int4 *>(child + 8 * (int64_t)c);
+        const int4 a = p[0], b = p[1];
+        const int v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        int cnt = 0, last = -1;
+        for (int s = 0; s < 8; ++s)
+            if (v[s] != -1) { ++cnt; last = v[s]; }
+        if (cnt == 1 && last >= n) {
+            c = last;
+            root = false;
+            continue;
+        }
+        const int lev = root ? 0 : cell_level[c];
+        return make_int2(block_of<BATCH>(c, n), cnt | (lev << 4));
+    }
+}
+
+template <bool BATCH>
+__global__ void __launch_bounds__(256) frame_layout_kernel(
+    int n, int nroot, const int *__restrict__ index, const int *__restrict__ child, const int32_t *__restrict__ pos,
+    const int32_t *__restrict__ cell_level, const float4 *__restrict__ nodes4, const float4 *__restrict__ bodies4,
+    float4 *__restrict__ fnode, int2 *__restrict__ fmeta, float4 *__restrict__ froot4, int2 *__restrict__ frootm,
+    int32_t *__restrict__ pblk, int32_t *__restrict__ bpar, const DeviceState *st)
 {
     if (latched(st)) return;
     const int ncell = *index - n;
     const int64_t items = BATCH ? ncell : ncell + 1;
     const int64_t total = 8 * items;
-    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < total;
+    const int64_t bound = (total + 63) & ~(int64_t)63;  // whole waves iterate together (ballot below)
+    const int lane = threadIdx.x & 63;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < bound;
          k += (int64_t)blockDim.x * gridDim.x) {
         const int64_t item = k >> 3;
-        const int64_t node = BATCH ? n + item : (item == 0 ? 0 : n + item - 1);
-        const int ch = child[8 * node + (k & 7)];
-        tchild[8 * node + (k & 7)] = (ch >= 0 && ch < n) ? pos[ch] : ch;
+        const int s = (int)(k & 7);
+        const int node = BATCH ? n + (int)item : (item == 0 ? 0 : n + (int)item - 1);
+        const int ch = k < total ? child[8 * (int64_t)node + s] : -1;
+        const uint64_t occ = __ballot(ch != -1);
+        const unsigned seg = (unsigned)(occ >> (lane & ~7)) & 0xffu;
+        const int rank = __popc(seg & ((1u << s) - 1u));
+        if (k < total && item < nroot && s == 0) {
+            froot4[item] = BATCH ? nodes4[node] : st->root4;
+            frootm[item] = chain_bottom<BATCH>(node, n, child, cell_level, true);
+            pblk[item] = -1;
+        }
+        if (ch == -1) continue;
+        const int64_t slot = 8 * item + rank;
+        if (ch < n) {
+            const int ps = pos[ch];
+            fnode[slot] = bodies4[ch];
+            fmeta[slot] = make_int2(ps, 0);
+            bpar[ps] = (int)item;
+        } else {
+            fnode[slot] = nodes4[ch];
+            fmeta[slot] = chain_bottom<BATCH>(ch, n, child, cell_level, false);
+            pblk[block_of<BATCH>(ch, n)] = (int)item;
+        }
     }
 }
 
@@ -612,8 +676,9 @@ void launch_layout(nbx_context *c, hipStream_t s, int *sorted)
 {
     const int n = c->n;
     layout_bodies_kernel<<<capped_grid(n), 256, 0, s>>>(n, c->perm, c->bodies4, c->nodes4, sorted, c->st);
-    layout_child_kernel<false><<<capped_grid(8 * (int64_t)(c->m - c->n + 1)), 256, 0, s>>>(
-        n, c->index, c->child, c->pos, c->tchild, c->st);
+    frame_layout_kernel<false><<<capped_grid(8 * (int64_t)(c->m - c->n + 1)), 256, 0, s>>>(
+        n, 1, c->index, c->child, c->pos, c->cell_level, c->nodes4, c->bodies4, c->fnode, c->fmeta, c->froot4,
+        c->frootm, c->pblk, c->bpar, c->st);
 }
 
 void launch_batch_build(nbx_batch *b, hipStream_t s)
@@ -634,8 +699,9 @@ void launch_batch_summarize(nbx_batch *b, hipStream_t s)
 void launch_batch_layout(nbx_batch *b, hipStream_t s)
 {
     layout_bodies_kernel<<<capped_grid(b->N), 256, 0, s>>>(b->N, b->perm, b->bodies4, b->nodes4, b->sorted, b->st);
-    layout_child_kernel<true><<<capped_grid(8 * (int64_t)(b->m - b->N)), 256, 0, s>>>(
-        b->N, b->index, b->child, b->pos, b->tchild, b->st);
+    frame_layout_kernel<true><<<capped_grid(8 * (int64_t)(b->m - b->N)), 256, 0, s>>>(
+        b->N, b->S, b->index, b->child, b->pos, b->cell_level, b->nodes4, b->bodies4, b->fnode, b->fmeta, b->froot4,
+        b->frootm, b->pblk, b->bpar, b->st);
 }
 
 }  // namespace nbx

@@ -6,6 +6,7 @@
 // device-side error latch.  No stage allocates memory; nbx_create does.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -123,7 +124,9 @@ int nbx_create(const nbx_params *p, int n_max, int m_max, int device, nbx_contex
               dalloc(&c->cell_level, m) == hipSuccess && dalloc(&c->parent, m) == hipSuccess &&
               dalloc(&c->arrive, m) == hipSuccess && dalloc(&c->mass64, m) == hipSuccess &&
               dalloc(&c->mom64, 3 * m) == hipSuccess && dalloc(&c->nodes4, m) == hipSuccess &&
-              dalloc(&c->tchild, 8 * m) == hipSuccess && dalloc(&c->counts, n) == hipSuccess &&
+              dalloc(&c->fnode, 8 * m) == hipSuccess && dalloc(&c->fmeta, 8 * m) == hipSuccess &&
+              dalloc(&c->froot4, 1) == hipSuccess && dalloc(&c->frootm, 1) == hipSuccess &&
+              dalloc(&c->pblk, m) == hipSuccess && dalloc(&c->bpar, n) == hipSuccess && dalloc(&c->counts, n) == hipSuccess &&
               dalloc(&c->acc_scratch, 3 * n) == hipSuccess && dalloc(&c->bodies4, n) == hipSuccess &&
               hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
     for (int k = 0; ok && k < 8; ++k) ok = hipEventCreate(&c->stage_ev[k]) == hipSuccess;
@@ -144,7 +147,7 @@ int nbx_destroy(nbx_context *c)
     hipSetDevice(c->device);
     hipDeviceSynchronize();
     void *bufs[] = {c->st, c->bbox_part, c->keys, c->skeys, c->kbuf, c->perm, c->vbuf, c->pos, c->hist,
-                    c->cell_level, c->parent, c->arrive, c->mass64, c->mom64, c->nodes4, c->tchild, c->counts,
+                    c->cell_level, c->parent, c->arrive, c->mass64, c->mom64, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->pblk, c->bpar, c->adj_tgt, c->adj_ab, c->counts,
                     c->acc_scratch, c->bodies4};
     for (void *b : bufs)
         if (b) hipFree(b);
@@ -356,6 +359,29 @@ int nbx_count_interactions(nbx_context *c, int n, int64_t *counts, void *stream)
     float *scratch = c->acc_scratch;
     StageTimer t(c, s, nullptr);
     nbx::launch_force(c, s, scratch, scratch + n, scratch + 2 * (size_t)n, 1.f, counts);
+    return t.finish();
+}
+
+int nbx_force_vjp(nbx_context *c, const float *ux, const float *uy, const float *uz, float g, float *gx, float *gy,
+                  float *gz, float *gm, void *stream, float *elapsed_ms)
+{
+    if (!c) return NBX_ERR_INVALID;
+    if (c->stage < 4) return fail(c, NBX_ERR_ORDER, "nbx_force_vjp needs nbx_sort first");
+    if (!ux || !uy || !uz || !gx || !gy || !gz || !gm) return fail(c, NBX_ERR_INVALID, "null array");
+    if (!std::isfinite(g)) return fail(c, NBX_ERR_INVALID, "g must be finite");
+    hipSetDevice(c->device);
+    if (!c->adj_tgt) {
+        const size_t cnt = 4 * ((size_t)c->m_max + 1);
+        if (dalloc(&c->adj_tgt, cnt) != hipSuccess || dalloc(&c->adj_ab, cnt) != hipSuccess) {
+            if (c->adj_tgt) hipFree(c->adj_tgt);
+            if (c->adj_ab) hipFree(c->adj_ab);
+            c->adj_tgt = c->adj_ab = nullptr;
+            return fail(c, NBX_ERR_HIP, "adjoint scratch allocation failed");
+        }
+    }
+    hipStream_t s = (hipStream_t)stream;
+    StageTimer t(c, s, elapsed_ms);
+    nbx::launch_vjp(c, s, ux, uy, uz, gx, gy, gz, gm, g);
     return t.finish();
 }
 

@@ -87,7 +87,16 @@ struct nbx_context {
     int32_t *cell_level = nullptr, *parent = nullptr, *arrive = nullptr;
     double *mass64 = nullptr, *mom64 = nullptr;
     float4 *nodes4 = nullptr;
-    int32_t *tchild = nullptr;
+    float4 *fnode = nullptr;     // [8 * m_max] frame blocks of the force walk: child (x, y, z, m)
+    int2 *fmeta = nullptr;       // [8 * m_max] child descriptors
+    float4 *froot4 = nullptr;    // root as a child of a virtual parent
+    int2 *frootm = nullptr;
+    int32_t *pblk = nullptr;     // [m_max] parent block of each block (-1: root)
+    int32_t *bpar = nullptr;     // [n_max] parent block of each body, by sorted position
+    // adjoint scratch (allocated by the first nbx_force_vjp)
+    double *adj_tgt = nullptr;   // [4 (n_max + m_max)] gradient sums of bodies (sorted), then blocks
+    double *adj_ab = nullptr;    // [4 m_max] per block: sum over ancestors of G/M and GM - G.X/M
+    float g_last = 1.f;          // gravity of the last nbx_compute_forces
     int64_t *counts = nullptr;
     float *acc_scratch = nullptr;
     float4 *bodies4 = nullptr;  // (x, y, z, m) of every body in input order
@@ -129,7 +138,11 @@ struct nbx_batch {
     int32_t *cell_level = nullptr, *parent = nullptr, *arrive = nullptr;
     double *mass64 = nullptr, *mom64 = nullptr;
     float4 *nodes4 = nullptr;
-    int32_t *tchild = nullptr;
+    float4 *fnode = nullptr;     // [8 * M] frame blocks
+    int2 *fmeta = nullptr;
+    float4 *froot4 = nullptr;    // [S] roots
+    int2 *frootm = nullptr;
+    int32_t *pblk = nullptr, *bpar = nullptr;
     int64_t *counts = nullptr;
     float *acc_scratch = nullptr;
     float4 *bodies4 = nullptr;
@@ -189,5 +202,9 @@ void launch_batch_build(nbx_batch *b, hipStream_t s);
 void launch_batch_summarize(nbx_batch *b, hipStream_t s);
 void launch_batch_layout(nbx_batch *b, hipStream_t s);
 void launch_batch_force(nbx_batch *b, hipStream_t s, int64_t *counts, int waves);
+
+// launchers (k_adjoint.hip)
+void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy, const float *uz, float *gx,
+                float *gy, float *gz, float *gm, float g);
 
 }  // namespace nbx
