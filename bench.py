@@ -65,6 +65,8 @@ def parse():
     ap.add_argument("--no-kahan", action="store_true")
     ap.add_argument("--skip", default="", help="comma list of sections to skip: plummer,batched,adjoint")
     ap.add_argument("--fd-coords", type=int, default=64, help="coordinates of the adjoint's central differences")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="MiB copied once by a plain device copy before the timed region (profiler calibration)")
     return ap.parse_args()
 
 
@@ -126,10 +128,11 @@ def run_plummer(args):
             "g_interactions_per_s_force": round(i_alg / force / 1e6, 2), "steps": 20}
 
 
-def run_batched(args, stream):
+def run_batched(args):
     """512 Plummer spheres of uneven N against one sphere of the same total N."""
     import torch
     import nns_amd
+    stream = torch.cuda.Stream()  # graph capture needs a stream of its own
     rng = np.random.default_rng(args.seed + 7)
     S = 512
     sizes = np.exp(rng.uniform(np.log(1024.0), np.log(65536.0), S)).astype(np.int32)
@@ -146,6 +149,7 @@ def run_batched(args, stream):
         b.graph_step(stream)
     torch.cuda.synchronize()
     c0 = int(b.count_interactions().sum().item())
+    torch.cuda.synchronize()
     K = 10
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -157,7 +161,8 @@ def run_batched(args, stream):
     b.check()
     ms_b = e0.elapsed_time(e1) / K
     c1 = int(b.count_interactions().sum().item())
-    stage_b = [b.step(stage_times=True) for _ in range(3)]
+    stage_b = [b.step(stage_times=True, stream=stream) for _ in range(3)]
+    torch.cuda.synchronize()
     b.check()
     i_b = 0.5 * (c0 + c1)
     del b
@@ -271,6 +276,12 @@ def main():
           flush=True)
 
     stream = torch.cuda.current_stream()
+    if args.copy_probe > 0:
+        src = torch.ones(args.copy_probe << 18, dtype=torch.float32, device=bh.dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(args.warmup):
         bh.stages()
     bh.ctx.check()
@@ -396,7 +407,7 @@ def main():
     if "plummer" not in skip:
         out["plummer"] = run_plummer(args)
     if "batched" not in skip:
-        out["batched"] = run_batched(args, stream)
+        out["batched"] = run_batched(args)
     if "adjoint" not in skip:
         out["adjoint"] = run_adjoint(args)
     print(json.dumps(out))
