@@ -30,7 +30,7 @@ template <int W>
 __global__ void __launch_bounds__(64 * W) vjp_kernel(
     int n, int ngroups, const float4 *__restrict__ nodes4, const float4 *__restrict__ fnode,
     const int2 *__restrict__ fmeta, const float4 *__restrict__ froot4, const int2 *__restrict__ frootm,
-    const int32_t *__restrict__ perm, const DeviceState *st, float theta, float eps2, float G,
+    const int32_t *__restrict__ perm, const DeviceState *st, float eps2, float G,
     const float *__restrict__ ux, const float *__restrict__ uy, const float *__restrict__ uz,
     float *__restrict__ gx, float *__restrict__ gy, float *__restrict__ gz, double *__restrict__ tgt,
     int *__restrict__ err)
@@ -46,17 +46,6 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
 
     const int first = g * 64;
     const int cnt_bodies = min(64, n - first);
-    const double lc0 = ldexp(st->ext[3] / kMaskCoord, kKeyLevels);
-    const double r0 = lc0 / (double)theta;
-    const float crit0 = (float)(r0 * r0);
-    const int cb0 = __float_as_int(crit0);
-    const int e0 = (cb0 >> 23) & 0xff;
-    auto crit_at = [&](const int lev) -> float {
-        int b;
-        if (e0 > 2 * lev && e0 < 255) b = cb0 - (lev << 24);
-        else b = __builtin_amdgcn_readfirstlane(__float_as_int(ldexpf(crit0, -2 * lev)));
-        return __int_as_float(b);
-    };
 
     const int p = first + lane;
     const bool valid = lane < cnt_bodies;
@@ -101,41 +90,33 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
         __builtin_amdgcn_wave_barrier();
     };
 
-    auto step = [&](const float4 q, const int dx_, const int dy_, const uint64_t mask) -> uint64_t {
+    // One child step (descriptor {tag, crit} as in the force walk).
+    auto step = [&](const float4 q, const int tag, const float crit, const uint64_t mask) -> uint64_t {
         const float dx = q.x - me.x;
         const float dy = q.y - me.y;
         const float dz = q.z - me.z;
         const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        uint64_t takem, omask;
-        if (dy_ == 0) {
-            takem = mask;
-            omask = 0;
-        } else {
-            const uint64_t ok = __ballot(crit_at(dy_ >> 4) < d2);
-            takem = mask & ok;
-            omask = mask & ~ok;
-        }
-        takem &= __ballot(d2 > 0.f);
+        const uint64_t ok = __ballot(crit < d2);
+        const uint64_t omask = mask & ~ok;
+        const uint64_t takem = mask & ok & __ballot(d2 > 0.f);
         if (takem) {
-            float hx = 0.f, hy = 0.f, hz = 0.f, hw = 0.f;
-            if (__builtin_amdgcn_inverse_ballot_w64(takem)) {
-                const float r2 = d2 + eps2;
-                const float rs = __builtin_amdgcn_rsqf(r2);
-                const float rs2 = rs * rs;
-                const float s3 = rs * rs2;
-                const float ud = fmaf(vz, dz, fmaf(vy, dy, vx * dx));
-                const float k3 = 3.f * ud * rs2;
-                const float ms = q.w * s3;
-                hx = ms * fmaf(-k3, dx, vx);
-                hy = ms * fmaf(-k3, dy, vy);
-                hz = ms * fmaf(-k3, dz, vz);
-                hw = s3 * ud;
-                ox -= hx;
-                oy -= hy;
-                oz -= hz;
-            }
+            const bool sel = __builtin_amdgcn_inverse_ballot_w64(takem);
+            const float r2 = sel ? d2 + eps2 : 1.f;  // finite on every lane
+            const float rs = __builtin_amdgcn_rsqf(r2);
+            const float rs2 = rs * rs;
+            const float s3 = sel ? rs * rs2 : 0.f;
+            const float ud = fmaf(vz, dz, fmaf(vy, dy, vx * dx));
+            const float k3 = 3.f * ud * rs2;
+            const float ms = q.w * s3;
+            const float hx = ms * fmaf(-k3, dx, vx);
+            const float hy = ms * fmaf(-k3, dy, vy);
+            const float hz = ms * fmaf(-k3, dz, vz);
+            const float hw = s3 * ud;
+            ox -= hx;
+            oy -= hy;
+            oz -= hz;
             rw[nrec][lane] = make_float4(hx, hy, hz, hw);
-            if (lane == 0) ri[nrec] = dy_ == 0 ? dx_ : n + dx_;
+            if (lane == 0) ri[nrec] = (tag & 15) == 0 ? (tag >> 4) : n + (tag >> 4);
             nrec = __builtin_amdgcn_readfirstlane(nrec + 1);
             if (nrec == kVjpBatch) {
                 flush(kVjpBatch);
@@ -149,19 +130,24 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
     int sp = 0;
     {
         const int2 rd = frootm[0];
-        const int rx = __builtin_amdgcn_readfirstlane(rd.x), ry = __builtin_amdgcn_readfirstlane(rd.y);
-        const uint64_t om = step(froot4[0], rx, ry, live);
+        const int rt = __builtin_amdgcn_readfirstlane(rd.x);
+        const float rc = __int_as_float(__builtin_amdgcn_readfirstlane(rd.y));
+        const uint64_t om = step(froot4[0], rt, rc, live);
         if (om) {
-            if (lane == 0) s[0] = make_uint4((unsigned)rx, (unsigned)ry, (unsigned)om, (unsigned)(om >> 32));
+            if (lane == 0) s[0] = make_uint4((unsigned)rt, 0u, (unsigned)om, (unsigned)(om >> 32));
             sp = 1;
         }
     }
-    bool overflow = false;
     while (sp > 0) {
+        if (sp > kForceStack - 7) {
+            if (lane == 0) atomicOr(err, kErrStack);
+            break;
+        }
         sp = __builtin_amdgcn_readfirstlane(sp - 1);
         const uint4 e = s[sp];
-        const int blk = __builtin_amdgcn_readfirstlane((int)e.x);
-        const int nch = __builtin_amdgcn_readfirstlane((int)e.y) & 15;
+        const int tag = __builtin_amdgcn_readfirstlane((int)e.x);
+        const int nch = tag & 15;
+        const int blk = tag >> 4;
         const uint64_t mask = ((uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)e.w) << 32) |
                               (uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)e.z);
         const float4 *bq = fnode + 8 * (int64_t)blk;
@@ -173,23 +159,25 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
             q[j] = bq[j];
             d[j] = bd[j];
         }
+        asm volatile("" ::"s"(q[0].x), "s"(q[0].y), "s"(q[0].z), "s"(q[0].w), "s"(q[1].x), "s"(q[1].y),
+                     "s"(q[1].z), "s"(q[1].w), "s"(q[2].x), "s"(q[2].y), "s"(q[2].z), "s"(q[2].w), "s"(q[3].x),
+                     "s"(q[3].y), "s"(q[3].z), "s"(q[3].w));
+        asm volatile("" ::"s"(q[4].x), "s"(q[4].y), "s"(q[4].z), "s"(q[4].w), "s"(q[5].x), "s"(q[5].y),
+                     "s"(q[5].z), "s"(q[5].w), "s"(q[6].x), "s"(q[6].y), "s"(q[6].z), "s"(q[6].w), "s"(q[7].x),
+                     "s"(q[7].y), "s"(q[7].z), "s"(q[7].w));
+        asm volatile("" ::"s"(d[0].x), "s"(d[0].y), "s"(d[1].x), "s"(d[1].y), "s"(d[2].x), "s"(d[2].y),
+                     "s"(d[3].x), "s"(d[3].y), "s"(d[4].x), "s"(d[4].y), "s"(d[5].x), "s"(d[5].y), "s"(d[6].x),
+                     "s"(d[6].y), "s"(d[7].x), "s"(d[7].y));
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             if (j >= nch) break;
-            const int cx_ = __builtin_amdgcn_readfirstlane(d[j].x), cy_ = __builtin_amdgcn_readfirstlane(d[j].y);
-            const uint64_t om = step(q[j], cx_, cy_, mask);
+            const int tj = __builtin_amdgcn_readfirstlane(d[j].x);
+            const float cj = __int_as_float(__builtin_amdgcn_readfirstlane(d[j].y));
+            const uint64_t om = step(q[j], tj, cj, mask);
             if (om) {
-                if (sp >= kForceStack) {
-                    overflow = true;
-                    break;
-                }
-                if (lane == 0) s[sp] = make_uint4((unsigned)cx_, (unsigned)cy_, (unsigned)om, (unsigned)(om >> 32));
+                if (lane == 0) s[sp] = make_uint4((unsigned)tj, 0u, (unsigned)om, (unsigned)(om >> 32));
                 sp = __builtin_amdgcn_readfirstlane(sp + 1);
             }
-        }
-        if (overflow) {
-            if (lane == 0) atomicOr(err, kErrStack);
-            break;
         }
     }
     if (nrec > 0) flush(nrec);
@@ -270,12 +258,13 @@ void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy,
 {
     const int n = c->n;
     if (n <= 0) return;
+    if (c->layout_theta != c->p.theta) launch_frame_layout(c, s);
     const int64_t ntgt = 4 * ((int64_t)n + (c->m - n + 1));
     clear_f64_kernel<<<(int)std::min<int64_t>((ntgt + 255) / 256, 65536), 256, 0, s>>>(c->adj_tgt, ntgt);
     constexpr int W = 2;
     const int ngroups = (n + 63) / 64;
     vjp_kernel<W><<<(ngroups + W - 1) / W, 64 * W, 0, s>>>(
-        n, ngroups, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->perm, c->st, c->p.theta,
+        n, ngroups, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->perm, c->st,
         c->p.softening * c->p.softening, g, ux, uy, uz, gx, gy, gz, c->adj_tgt, &c->st->err);
     const int64_t nblk = c->m - n + 1;
     adj_cells_kernel<<<(int)std::min<int64_t>((nblk + 255) / 256, 65536), 256, 0, s>>>(

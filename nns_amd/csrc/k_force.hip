@@ -12,11 +12,11 @@
 // an LDS stack of {block, nch | level << 4, lane mask}.
 //
 // Each lane applies its own opening test to a cell child (l/D < theta written
-// as crit_L < |d|^2 with crit_L = (l_L/theta)^2; the reference CPU test of
-// 3D/OpenMP/OpenMP_NBody/src/Interaction.cpp:64-67).  The test is a ballot, so
-// lane masks stay in SGPRs: lanes that accept add the cell's monopole under
-// exec = mask & accept, lanes that reject push the child's frame with
-// mask & ~accept.  Every lane therefore sees exactly its own per-particle
+// as crit_L < |d|^2 with crit_L = (l_L/theta)^2, the reference CPU test of
+// 3D/OpenMP/OpenMP_NBody/src/Interaction.cpp:64-67; crit_L comes with the
+// child's descriptor).  The test is a ballot, so lane masks stay in SGPRs:
+// lanes that accept keep the cell's monopole (a select on mask & accept),
+// lanes that reject push the child's frame with mask & ~accept.  Every lane therefore sees exactly its own per-particle
 // interaction list; the wave pays for the union of the 64 lists.  Bodies are
 // always direct (Interaction.cpp:60-63); a zero distance contributes nothing
 // (:28-30).
@@ -24,15 +24,17 @@
 // Interaction: a += m d (|d|^2 + eps^2)^-3/2, d = q - x (the 20-flop form of
 // GPU Gems 3 ch. 31), with the raw v_rsq_f32 (|d|^2 + eps^2 is never denormal
 // when eps > 0; the unsoftened variant masks d = 0).  With Kahan on, monopoles
-// go into a plain partial that is folded into a compensated total every 16
-// children.
+// go into a plain partial that is folded into a compensated total every 4
+// frames.  Bookkeeping that is not per child (stack bound, Kahan) is per frame:
+// the scalar unit, shared by the CU's four SIMDs, is the other limit here.
 //
 // Blocks are remapped so that the blocks one XCD receives (dispatch is
 // round-robin over the 8 XCDs) walk consecutive groups and share its L2.
 //
 // Single system: group g is sorted positions [64g, 64g+64) and the walk starts
 // from froot[0].  Ensemble: group g comes from a table {first, count, system};
-// the walk starts at froot[system] with the system's own theta and extent.
+// the walk starts at froot[system] (thresholds of the system's own theta and
+// extent are in the descriptors).
 #include "nbx_device.h"
 
 namespace nbx {
@@ -41,10 +43,9 @@ template <bool KAHAN, bool COUNT, bool SOFT, bool BATCH, int W>
 __global__ void __launch_bounds__(64 * W) force_kernel(
     int n, int ngroups, const float4 *__restrict__ nodes4, const float4 *__restrict__ fnode,
     const int2 *__restrict__ fmeta, const float4 *__restrict__ froot4, const int2 *__restrict__ frootm,
-    const int32_t *__restrict__ perm, const DeviceState *st, float theta, float eps2, float G,
-    float *__restrict__ ax, float *__restrict__ ay, float *__restrict__ az, int64_t *__restrict__ counts,
-    int *__restrict__ err, const int4 *__restrict__ groups, const float *__restrict__ sys_theta,
-    const double *__restrict__ sys_ext)
+    const int32_t *__restrict__ perm, const DeviceState *st, float eps2, float G, float *__restrict__ ax,
+    float *__restrict__ ay, float *__restrict__ az, int64_t *__restrict__ counts, int *__restrict__ err,
+    const int4 *__restrict__ groups)
 {
     __shared__ uint4 stk[W][kForceStack];
     if (latched(st)) return;
@@ -54,27 +55,16 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
     if (g >= ngroups) return;
 
     int first, cnt_bodies, sys;
-    double span;
-    float th;
     if (BATCH) {
         const int4 gr = groups[g];
         first = gr.x;
         cnt_bodies = gr.y;
         sys = gr.z;
-        span = sys_ext[4 * sys + 3];
-        th = sys_theta[sys];
     } else {
         first = g * 64;
         cnt_bodies = min(64, n - first);
         sys = 0;
-        span = st->ext[3];
-        th = theta;
     }
-    // crit_0 = (l_0/theta)^2, l_0 = side of the root cell of the key grid
-    const double lc0 = ldexp(span / kMaskCoord, kKeyLevels);
-    const double r0 = lc0 / (double)th;
-    const float crit0 = (float)(r0 * r0);
-
     const int p = first + lane;
     const bool valid = lane < cnt_bodies;
     const float4 me = valid ? nodes4[p] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -86,53 +76,29 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
     int64_t cnt = 0;
     int tile = 0;
 
-    // crit_L = crit_0 4^-L: subtracting 2L from the exponent field is exact
-    // while the result stays normal; otherwise ldexpf.
-    const int cb0 = __float_as_int(crit0);
-    const int e0 = (cb0 >> 23) & 0xff;
-    auto crit_at = [&](const int lev) -> float {
-        int b;
-        if (e0 > 2 * lev && e0 < 255) b = cb0 - (lev << 24);
-        else b = __builtin_amdgcn_readfirstlane(__float_as_int(ldexpf(crit0, -2 * lev)));
-        return __int_as_float(b);
-    };
-
     // One child step: every lane of `mask` meets child q with descriptor
-    // (dx_, dy_); returns the lanes that open it.
-    auto step = [&](const float4 q, const int dx_, const int dy_, const uint64_t mask) -> uint64_t {
+    // {tag, crit}; returns the lanes that open it.  The monopole is evaluated
+    // on every lane and kept where the lane takes the child (a select, not a
+    // branch); a body's crit is -1, so it is always taken and never opened.
+    auto step = [&](const float4 q, const int tag, const float crit, const uint64_t mask) -> uint64_t {
         const float dx = q.x - me.x;
         const float dy = q.y - me.y;
         const float dz = q.z - me.z;
         const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        uint64_t takem, omask;
-        if (dy_ == 0) {
-            takem = mask;
-            omask = 0;
-        } else {
-            const uint64_t ok = __ballot(crit_at(dy_ >> 4) < d2);
-            takem = mask & ok;
-            omask = mask & ~ok;
-        }
+        const uint64_t ok = __ballot(crit < d2);
+        uint64_t takem = mask & ok;
+        const uint64_t omask = mask & ~ok;
         if (COUNT) {
-            if (__builtin_amdgcn_inverse_ballot_w64(takem)) cnt += (dy_ == 0 && dx_ == p) ? 0 : 1;
+            if (__builtin_amdgcn_inverse_ballot_w64(takem)) cnt += ((tag & 15) == 0 && (tag >> 4) == p) ? 0 : 1;
         }
         if (!SOFT) takem &= __ballot(d2 > 0.f);
-        if (__builtin_amdgcn_inverse_ballot_w64(takem)) {
-            const float r2 = d2 + eps2;
-            const float ri = __builtin_amdgcn_rsqf(r2);
-            const float f = q.w * ri * (ri * ri);
-            px = fmaf(f, dx, px);
-            py = fmaf(f, dy, py);
-            pz = fmaf(f, dz, pz);
-        }
-        if (KAHAN) {
-            tile = __builtin_amdgcn_readfirstlane(tile + 1);
-            if (tile == 16) {
-                kahan_add(sx, cx, px); kahan_add(sy, cy, py); kahan_add(sz, cz, pz);
-                px = py = pz = 0.f;
-                tile = 0;
-            }
-        }
+        const float r2 = d2 + eps2;
+        const float ri = __builtin_amdgcn_rsqf(r2);
+        float f = q.w * ri * (ri * ri);
+        f = __builtin_amdgcn_inverse_ballot_w64(takem) ? f : 0.f;
+        px = fmaf(f, dx, px);
+        py = fmaf(f, dy, py);
+        pz = fmaf(f, dz, pz);
         return omask;
     };
 
@@ -140,19 +106,25 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
     int sp = 0;
     {
         const int2 rd = frootm[sys];
-        const int rx = __builtin_amdgcn_readfirstlane(rd.x), ry = __builtin_amdgcn_readfirstlane(rd.y);
-        const uint64_t om = step(froot4[sys], rx, ry, live);
+        const int rt = __builtin_amdgcn_readfirstlane(rd.x);
+        const float rc = __int_as_float(__builtin_amdgcn_readfirstlane(rd.y));
+        const uint64_t om = step(froot4[sys], rt, rc, live);
         if (om) {
-            if (lane == 0) s[0] = make_uint4((unsigned)rx, (unsigned)ry, (unsigned)om, (unsigned)(om >> 32));
+            if (lane == 0) s[0] = make_uint4((unsigned)rt, 0u, (unsigned)om, (unsigned)(om >> 32));
             sp = 1;
         }
     }
-    bool overflow = false;
     while (sp > 0) {
+        // a frame pops one entry and pushes at most 8
+        if (sp > kForceStack - 7) {
+            if (lane == 0) atomicOr(err, kErrStack);
+            break;
+        }
         sp = __builtin_amdgcn_readfirstlane(sp - 1);
         const uint4 e = s[sp];
-        const int blk = __builtin_amdgcn_readfirstlane((int)e.x);
-        const int nch = __builtin_amdgcn_readfirstlane((int)e.y) & 15;
+        const int tag = __builtin_amdgcn_readfirstlane((int)e.x);
+        const int nch = tag & 15;
+        const int blk = tag >> 4;
         const uint64_t mask = ((uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)e.w) << 32) |
                               (uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)e.z);
         const float4 *bq = fnode + 8 * (int64_t)blk;
@@ -177,20 +149,21 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             if (j >= nch) break;
-            const int cx_ = __builtin_amdgcn_readfirstlane(d[j].x), cy_ = __builtin_amdgcn_readfirstlane(d[j].y);
-            const uint64_t om = step(q[j], cx_, cy_, mask);
+            const int tj = __builtin_amdgcn_readfirstlane(d[j].x);
+            const float cj = __int_as_float(__builtin_amdgcn_readfirstlane(d[j].y));
+            const uint64_t om = step(q[j], tj, cj, mask);
             if (om) {
-                if (sp >= kForceStack) {
-                    overflow = true;
-                    break;
-                }
-                if (lane == 0) s[sp] = make_uint4((unsigned)cx_, (unsigned)cy_, (unsigned)om, (unsigned)(om >> 32));
+                if (lane == 0) s[sp] = make_uint4((unsigned)tj, 0u, (unsigned)om, (unsigned)(om >> 32));
                 sp = __builtin_amdgcn_readfirstlane(sp + 1);
             }
         }
-        if (overflow) {
-            if (lane == 0) atomicOr(err, kErrStack);
-            break;
+        if (KAHAN) {  // fold the partials every 4 frames (about 20 children)
+            tile = __builtin_amdgcn_readfirstlane(tile + 1);
+            if (tile == 4) {
+                kahan_add(sx, cx, px); kahan_add(sy, cy, py); kahan_add(sz, cz, pz);
+                px = py = pz = 0.f;
+                tile = 0;
+            }
         }
     }
     if (KAHAN) {
@@ -213,13 +186,11 @@ struct ForceArgs {
     const int2 *fmeta, *frootm;
     const int32_t *perm;
     const DeviceState *st;
-    float theta, eps2, G;
+    float eps2, G;
     float *ax, *ay, *az;
     int64_t *counts;
     int *err;
     const int4 *groups;
-    const float *sys_theta;
-    const double *sys_ext;
 };
 
 template <bool KAHAN, bool COUNT, bool SOFT, bool BATCH, int W>
@@ -227,8 +198,8 @@ static void launch_one(const ForceArgs &a, hipStream_t s)
 {
     const int nblk = (a.ngroups + W - 1) / W;
     force_kernel<KAHAN, COUNT, SOFT, BATCH, W><<<nblk, 64 * W, 0, s>>>(
-        a.n, a.ngroups, a.nodes4, a.fnode, a.fmeta, a.froot4, a.frootm, a.perm, a.st, a.theta, a.eps2, a.G, a.ax, a.ay, a.az, a.counts, a.err,
-        a.groups, a.sys_theta, a.sys_ext);
+        a.n, a.ngroups, a.nodes4, a.fnode, a.fmeta, a.froot4, a.frootm, a.perm, a.st, a.eps2, a.G, a.ax, a.ay, a.az,
+        a.counts, a.err, a.groups);
 }
 
 template <bool KAHAN, bool COUNT, bool SOFT, bool BATCH>
@@ -262,7 +233,6 @@ static ForceArgs single_args(nbx_context *c, float *ax, float *ay, float *az, fl
     a.ngroups = (c->n + 63) / 64;
     a.nodes4 = c->nodes4; a.fnode = c->fnode; a.fmeta = c->fmeta; a.froot4 = c->froot4; a.frootm = c->frootm;
     a.perm = c->perm; a.st = c->st;
-    a.theta = c->p.theta;
     a.eps2 = c->p.softening * c->p.softening;
     a.G = g;
     a.ax = ax; a.ay = ay; a.az = az; a.counts = counts;
@@ -273,12 +243,14 @@ static ForceArgs single_args(nbx_context *c, float *ax, float *ay, float *az, fl
 void launch_force(nbx_context *c, hipStream_t s, float *ax, float *ay, float *az, float g, int64_t *counts)
 {
     if (c->n <= 0) return;
+    if (c->layout_theta != c->p.theta) launch_frame_layout(c, s);  // thresholds of the current theta
     dispatch<false>(single_args(c, ax, ay, az, g, counts), s, c->p.kahan != 0, c->force_waves);
 }
 
 void launch_force_waves(nbx_context *c, hipStream_t s, float *ax, float *ay, float *az, float g, int waves)
 {
     if (c->n <= 0) return;
+    if (c->layout_theta != c->p.theta) launch_frame_layout(c, s);
     dispatch<false>(single_args(c, ax, ay, az, g, nullptr), s, c->p.kahan != 0, waves);
 }
 
@@ -289,7 +261,6 @@ void launch_batch_force(nbx_batch *b, hipStream_t s, int64_t *counts, int waves)
     a.ngroups = b->ngroups;
     a.nodes4 = b->nodes4; a.fnode = b->fnode; a.fmeta = b->fmeta; a.froot4 = b->froot4; a.frootm = b->frootm;
     a.perm = b->perm; a.st = b->st;
-    a.theta = b->p.theta;
     a.eps2 = b->p.softening * b->p.softening;
     a.G = b->p.gravity;
     a.ax = counts ? b->acc_scratch : b->ax;
@@ -298,8 +269,6 @@ void launch_batch_force(nbx_batch *b, hipStream_t s, int64_t *counts, int waves)
     a.counts = counts;
     a.err = &b->st->err;
     a.groups = b->groups;
-    a.sys_theta = b->theta;
-    a.sys_ext = b->ext;
     dispatch<true>(a, s, b->p.kahan != 0, waves);
 }
 

@@ -512,13 +512,17 @@ __global__ void __launch_bounds__(256) layout_bodies_kernel(int n, const int32_t
 
 // Frame blocks.  Block b lists the non-empty children of one cell, compacted in
 // slot order: fnode[8b + k] = (x, y, z, m) of the child and fmeta[8b + k] its
-// descriptor, {sorted position, 0} for a body and {block, nch | level << 4}
-// for a cell.  A cell child is described by the bottom of its chain, the first
-// descendant with two or more children: the cells of a one-child chain carry
-// the same mass and centre of mass (the FP64 sums of a single child are exact
-// copies), and crit_L = crit_0 4^-L falls with L, so a body accepts some cell
-// of the chain exactly when it accepts the bottom one, and meets the same
-// monopole once either way.  fnode keeps the data of the child itself.
+// descriptor {tag, crit}:
+//   body:  tag = sorted position << 4,        crit = -1 (always accepted);
+//   cell:  tag = block << 4 | children (1-8), crit = crit_L = (l_L/theta)^2,
+// with crit_L computed as in the opening test (crit_0 rounded once from FP64,
+// scaled by 4^-L exactly), so the walk compares crit < |d|^2 directly.
+// A cell child is described by the bottom of its chain, the first descendant
+// with two or more children: the cells of a one-child chain carry the same mass
+// and centre of mass (the FP64 sums of a single child are exact copies), and
+// crit_L falls with L, so a body accepts some cell of the chain exactly when it
+// accepts the bottom one, and meets the same monopole once either way.  fnode
+// keeps the data of the child itself.
 // Single system: block 0 is the root (slots child[0..7]), block i >= 1 is cell
 // n + i - 1.  Ensemble: block i is cell n + i, the roots being blocks 0..S-1.
 // froot4 / frootm describe each root as a child of a virtual parent.  The
@@ -530,11 +534,20 @@ __device__ inline int block_of(int c, int n)
     return BATCH ? c - n : (c == 0 ? 0 : c - n + 1);
 }
 
-// Bottom of the one-child chain starting at cell c (c = 0 is the single-system
-// root); returns {block, nch | level << 4}.
+// Opening threshold of a level-L cell: crit_L = (l_L/theta)^2, l_0 the side of the
+// root cell of the key grid.
+__device__ inline float crit_of(double span, float theta, int lev)
+{
+    const double lc0 = ldexp(span / kMaskCoord, kKeyLevels);
+    const double r0 = lc0 / (double)theta;
+    return ldexpf((float)(r0 * r0), -2 * lev);
+}
+
+// Descriptor of the bottom of the one-child chain starting at cell c (c = 0 is
+// the single-system root).
 template <bool BATCH>
 __device__ inline int2 chain_bottom(int c, int n, const int *__restrict__ child,
-                                    const int32_t *__restrict__ cell_level, bool root)
+                                    const int32_t *__restrict__ cell_level, bool root, double span, float theta)
 {
     for (;;) {
         const int4 *p = reinterpret_cast<const int4 *>(child + 8 * (int64_t)c);
@@ -549,16 +562,23 @@ __device__ inline int2 chain_bottom(int c, int n, const int *__restrict__ child,
             continue;
         }
         const int lev = root ? 0 : cell_level[c];
-        return make_int2(block_of<BATCH>(c, n), cnt | (lev << 4));
+        return make_int2((block_of<BATCH>(c, n) << 4) | cnt, __float_as_int(crit_of(span, theta, lev)));
     }
 }
+
+// Ensemble: the system of a cell from its first sorted body.
+struct SysInfo {
+    const int32_t *start, *perm, *sys_of;
+    const float *theta;
+    const double *ext;
+};
 
 template <bool BATCH>
 __global__ void __launch_bounds__(256) frame_layout_kernel(
     int n, int nroot, const int *__restrict__ index, const int *__restrict__ child, const int32_t *__restrict__ pos,
     const int32_t *__restrict__ cell_level, const float4 *__restrict__ nodes4, const float4 *__restrict__ bodies4,
     float4 *__restrict__ fnode, int2 *__restrict__ fmeta, float4 *__restrict__ froot4, int2 *__restrict__ frootm,
-    int32_t *__restrict__ pblk, int32_t *__restrict__ bpar, const DeviceState *st)
+    int32_t *__restrict__ pblk, int32_t *__restrict__ bpar, float theta, SysInfo si, const DeviceState *st)
 {
     if (latched(st)) return;
     const int ncell = *index - n;
@@ -566,6 +586,7 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
     const int64_t total = 8 * items;
     const int64_t bound = (total + 63) & ~(int64_t)63;  // whole waves iterate together (ballot below)
     const int lane = threadIdx.x & 63;
+    const int body_crit = __float_as_int(-1.f);
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < bound;
          k += (int64_t)blockDim.x * gridDim.x) {
         const int64_t item = k >> 3;
@@ -575,9 +596,20 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
         const uint64_t occ = __ballot(ch != -1);
         const unsigned seg = (unsigned)(occ >> (lane & ~7)) & 0xffu;
         const int rank = __popc(seg & ((1u << s) - 1u));
+        if (ch == -1 && !(k < total && item < nroot && s == 0)) continue;
+        double span;
+        float th;
+        if (BATCH) {
+            const int sys = item < nroot ? (int)item : si.sys_of[si.perm[si.start[node]]];
+            span = si.ext[4 * sys + 3];
+            th = si.theta[sys];
+        } else {
+            span = st->ext[3];
+            th = theta;
+        }
         if (k < total && item < nroot && s == 0) {
             froot4[item] = BATCH ? nodes4[node] : st->root4;
-            frootm[item] = chain_bottom<BATCH>(node, n, child, cell_level, true);
+            frootm[item] = chain_bottom<BATCH>(node, n, child, cell_level, true, span, th);
             pblk[item] = -1;
         }
         if (ch == -1) continue;
@@ -585,11 +617,11 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
         if (ch < n) {
             const int ps = pos[ch];
             fnode[slot] = bodies4[ch];
-            fmeta[slot] = make_int2(ps, 0);
+            fmeta[slot] = make_int2(ps << 4, body_crit);
             bpar[ps] = (int)item;
         } else {
             fnode[slot] = nodes4[ch];
-            fmeta[slot] = chain_bottom<BATCH>(ch, n, child, cell_level, false);
+            fmeta[slot] = chain_bottom<BATCH>(ch, n, child, cell_level, false, span, th);
             pblk[block_of<BATCH>(ch, n)] = (int)item;
         }
     }
@@ -676,9 +708,16 @@ void launch_layout(nbx_context *c, hipStream_t s, int *sorted)
 {
     const int n = c->n;
     layout_bodies_kernel<<<capped_grid(n), 256, 0, s>>>(n, c->perm, c->bodies4, c->nodes4, sorted, c->st);
+    launch_frame_layout(c, s);
+}
+
+void launch_frame_layout(nbx_context *c, hipStream_t s)
+{
+    SysInfo si{};
     frame_layout_kernel<false><<<capped_grid(8 * (int64_t)(c->m - c->n + 1)), 256, 0, s>>>(
-        n, 1, c->index, c->child, c->pos, c->cell_level, c->nodes4, c->bodies4, c->fnode, c->fmeta, c->froot4,
-        c->frootm, c->pblk, c->bpar, c->st);
+        c->n, 1, c->index, c->child, c->pos, c->cell_level, c->nodes4, c->bodies4, c->fnode, c->fmeta, c->froot4,
+        c->frootm, c->pblk, c->bpar, c->p.theta, si, c->st);
+    c->layout_theta = c->p.theta;
 }
 
 void launch_batch_build(nbx_batch *b, hipStream_t s)
@@ -699,9 +738,10 @@ void launch_batch_summarize(nbx_batch *b, hipStream_t s)
 void launch_batch_layout(nbx_batch *b, hipStream_t s)
 {
     layout_bodies_kernel<<<capped_grid(b->N), 256, 0, s>>>(b->N, b->perm, b->bodies4, b->nodes4, b->sorted, b->st);
+    SysInfo si{b->start, b->perm, b->sys_of, b->theta, b->ext};
     frame_layout_kernel<true><<<capped_grid(8 * (int64_t)(b->m - b->N)), 256, 0, s>>>(
         b->N, b->S, b->index, b->child, b->pos, b->cell_level, b->nodes4, b->bodies4, b->fnode, b->fmeta, b->froot4,
-        b->frootm, b->pblk, b->bpar, b->st);
+        b->frootm, b->pblk, b->bpar, 0.f, si, b->st);
 }
 
 }  // namespace nbx

@@ -96,7 +96,7 @@ struct nbx_context {
     // adjoint scratch (allocated by the first nbx_force_vjp)
     double *adj_tgt = nullptr;   // [4 (n_max + m_max)] gradient sums of bodies (sorted), then blocks
     double *adj_ab = nullptr;    // [4 m_max] per block: sum over ancestors of G/M and GM - G.X/M
-    float g_last = 1.f;          // gravity of the last nbx_compute_forces
+    float layout_theta = -1.f;   // theta the frame blocks' thresholds were computed with
     int64_t *counts = nullptr;
     float *acc_scratch = nullptr;
     float4 *bodies4 = nullptr;  // (x, y, z, m) of every body in input order
@@ -188,6 +188,8 @@ void launch_radix_passes(uint32_t *hist, hipStream_t s, const uint64_t *keys_in,
 void launch_build(nbx_context *c, hipStream_t s, int n, int m, int *child, int *index);
 void launch_summarize(nbx_context *c, hipStream_t s);
 void launch_layout(nbx_context *c, hipStream_t s, int *sorted);
+// Frame blocks of the force walk for the current theta (part of launch_layout).
+void launch_frame_layout(nbx_context *c, hipStream_t s);
 
 // launchers (k_force.hip)
 void launch_force(nbx_context *c, hipStream_t s, float *ax, float *ay, float *az, float g, int64_t *counts);
