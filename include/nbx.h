@@ -199,7 +199,7 @@ int nbx_autotune(nbx_context *ctx, void *stream, int *waves_out);
  * Each body meets exactly the nodes of its forward interaction list; cell gradients
  * flow down to the bodies through the centre-of-mass sums.  Node sums use FP64
  * atomics, so the last bits may differ between runs.  The first call allocates
- * 64 (m_max + 1) bytes of scratch. */
+ * 288 (m_max + 1) bytes of scratch (eight copies of the node sums, one per XCD). */
 int nbx_force_vjp(nbx_context *ctx, const float *ux, const float *uy, const float *uz, float g, float *gx,
                   float *gy, float *gz, float *gm, void *stream, float *elapsed_ms);
 

@@ -9,13 +9,15 @@
 // A zero distance contributes nothing (the forward skips it).  The node terms
 // of one child step are summed over the lanes that took it, through LDS: each
 // step writes one float4 per lane (rows padded to 65 entries, so the
-// transposed reads below are free of bank conflicts), and every kVjpBatch
-// recorded steps the wave reads them back transposed (8 lanes per step, 8
-// entries each), folds the 8 partial sums with three DPP moves and adds one
+// transposed reads below are free of bank conflicts), and every kVjpBatch = 4
+// recorded steps the wave reads them back transposed (16 lanes per step, 4
+// entries each), folds the 16 partial sums with four DPP moves and adds one
 // FP64 atomic per component and node.  Bodies are nodes too: their sums go to
-// the body's own entry.  The root and the top cells are accepted by nearly
-// every group, so their atomics are spread over kHotCopies copies (picked by
-// workgroup) that the flow-down pass adds up.
+// the body's own entry.  LDS stays under 7 KB per wave, so 16 waves fit a CU.
+// The sums have one copy per XCD (the wave's XCC_ID hardware register picks
+// it): a line that only one XCD updates stays in that XCD's L2.  The copy is a
+// placement choice, not a correctness one -- every copy takes device-scope
+// atomics -- and the flow-down pass adds the copies up first.
 //
 // A cell's X and M are sums over its bodies, X = sum m_k x_k / M, so the node
 // gradients flow down: for every body k
@@ -28,16 +30,24 @@
 
 namespace nbx {
 
-constexpr int kVjpBatch = 8;
-constexpr int kHotCopies = 16;
+constexpr int kVjpBatch = 4;
+constexpr int kXcdCopies = 8;
 
-// Sum over each aligned group of 8 lanes, in every lane of the group.
-__device__ inline float seg8_sum(float v)
+// Sum over each aligned group of 16 lanes (a DPP row), in every lane of the row.
+__device__ inline float seg16_sum(float v)
 {
     v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
     v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
     v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xF, 0xF, false));  // row_mirror
     return v;
+}
+
+__device__ inline int xcd_copy()
+{
+    unsigned v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return (int)(v & (kXcdCopies - 1));
 }
 
 template <int W>
@@ -47,7 +57,7 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
     const int32_t *__restrict__ perm, const DeviceState *st, float eps2, float G,
     const float *__restrict__ ux, const float *__restrict__ uy, const float *__restrict__ uz,
     float *__restrict__ gx, float *__restrict__ gy, float *__restrict__ gz, double *__restrict__ tgt,
-    double *__restrict__ hot_tgt, int hot_cap, int *__restrict__ err)
+    int64_t copy_stride, int *__restrict__ err)
 {
     __shared__ uint4 stk[W][kForceStack];
     __shared__ float4 red[W][kVjpBatch][65];
@@ -60,8 +70,7 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
 
     const int first = g * 64;
     const int cnt_bodies = min(64, n - first);
-    const int hot = min(st->ntop + 1, hot_cap);  // root and top cells: blocks [0, hot)
-    double *const hot_copy = hot_tgt + 4 * (int64_t)hot_cap * (blockIdx.x % kHotCopies);
+    double *const tcopy = tgt + copy_stride * xcd_copy();
 
     const int p = first + lane;
     const bool valid = lane < cnt_bodies;
@@ -77,25 +86,24 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
     float4(*rw)[65] = red[wave];
     int *ri = rid[wave];
 
-    // Lane 8s + q sums entries 8q .. 8q+7 of recorded step s; lane 8s adds the total.
+    // Lane 16s + q sums entries 4q .. 4q+3 of recorded step s; lane 16s adds the total.
     auto flush = [&](const int count) {
         __builtin_amdgcn_wave_barrier();
-        const int s_ = lane >> 3, q_ = lane & 7;
+        const int s_ = lane >> 4, q_ = lane & 15;
         float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
         if (s_ < count) {
 #pragma unroll
-            for (int l = 0; l < 8; ++l) {
-                const float4 v = rw[s_][8 * q_ + l];
+            for (int l = 0; l < 4; ++l) {
+                const float4 v = rw[s_][4 * q_ + l];
                 ax += v.x; ay += v.y; az += v.z; aw += v.w;
             }
         }
-        ax = seg8_sum(ax);
-        ay = seg8_sum(ay);
-        az = seg8_sum(az);
-        aw = seg8_sum(aw);
+        ax = seg16_sum(ax);
+        ay = seg16_sum(ay);
+        az = seg16_sum(az);
+        aw = seg16_sum(aw);
         if (q_ == 0 && s_ < count) {
-            const int id = ri[s_];
-            double *t = (id >= n && id - n < hot) ? hot_copy + 4 * (int64_t)(id - n) : tgt + 4 * (int64_t)id;
+            double *t = tcopy + 4 * (int64_t)ri[s_];
             unsafeAtomicAdd(t + 0, (double)ax);
             unsafeAtomicAdd(t + 1, (double)ay);
             unsafeAtomicAdd(t + 2, (double)az);
@@ -209,16 +217,15 @@ __global__ void __launch_bounds__(256) clear_f64_kernel(double *__restrict__ p, 
 }
 
 // Per block b: A and B summed over b and its ancestors (walk up pblk).
-// Folds the copies of the hot blocks into the main sums (blocks [0, hot)).
-__global__ void __launch_bounds__(256) adj_hot_kernel(int n, const double *__restrict__ hot_tgt, int hot_cap,
-                                                      double *__restrict__ tgt, const DeviceState *st)
+// Adds the per-XCD copies into copy 0 (count values each).
+__global__ void __launch_bounds__(256) adj_fold_kernel(double *__restrict__ tgt, int64_t count, int64_t copy_stride,
+                                                       const DeviceState *st)
 {
     if (latched(st)) return;
-    const int hot = min(st->ntop + 1, hot_cap);
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 4 * hot; k += blockDim.x * gridDim.x) {
-        double v = tgt[4 * (int64_t)n + k];
-        for (int c = 0; c < kHotCopies; ++c) v += hot_tgt[4 * (int64_t)hot_cap * c + k];
-        tgt[4 * (int64_t)n + k] = v;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < count; k += (int64_t)blockDim.x * gridDim.x) {
+        double v = tgt[k];
+        for (int c = 1; c < kXcdCopies; ++c) v += tgt[copy_stride * c + k];
+        tgt[k] = v;
     }
 }
 
@@ -280,13 +287,7 @@ __global__ void __launch_bounds__(256) adj_bodies_kernel(int n, const int32_t *_
     }
 }
 
-int vjp_hot_capacity(int n_max)
-{
-    const int samples = (n_max + kSubtreeMax - 1) / kSubtreeMax;
-    return kLevels * samples + 1;  // root plus every possible top cell
-}
-
-int vjp_hot_copies() { return kHotCopies; }
+int vjp_copies() { return kXcdCopies; }
 
 void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy, const float *uz, float *gx,
                 float *gy, float *gz, float *gm, float g)
@@ -294,18 +295,18 @@ void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy,
     const int n = c->n;
     if (n <= 0) return;
     if (c->layout_theta != c->p.theta) launch_frame_layout(c, s);
-    const int64_t ntgt = 4 * ((int64_t)n + (c->m - n + 1));
-    clear_f64_kernel<<<(int)std::min<int64_t>((ntgt + 255) / 256, 65536), 256, 0, s>>>(c->adj_tgt, ntgt);
-    const int64_t nhot = 4 * (int64_t)c->adj_hot_cap * kHotCopies;
-    clear_f64_kernel<<<(int)std::min<int64_t>((nhot + 255) / 256, 65536), 256, 0, s>>>(c->adj_hot, nhot);
+    const int64_t ntgt = 4 * ((int64_t)n + (c->m - n + 1));  // bodies, then blocks
+    const int64_t stride = 4 * ((int64_t)c->m_max + 1);      // one copy
+    for (int k = 0; k < kXcdCopies; ++k)
+        clear_f64_kernel<<<(int)std::min<int64_t>((ntgt + 255) / 256, 65536), 256, 0, s>>>(c->adj_tgt + stride * k,
+                                                                                           ntgt);
     constexpr int W = 2;
     const int ngroups = (n + 63) / 64;
     vjp_kernel<W><<<(ngroups + W - 1) / W, 64 * W, 0, s>>>(
         n, ngroups, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->perm, c->st,
-        c->p.softening * c->p.softening, g, ux, uy, uz, gx, gy, gz, c->adj_tgt, c->adj_hot, c->adj_hot_cap,
-        &c->st->err);
-    adj_hot_kernel<<<(int)std::min<int64_t>((4 * (int64_t)c->adj_hot_cap + 255) / 256, 65536), 256, 0, s>>>(
-        n, c->adj_hot, c->adj_hot_cap, c->adj_tgt, c->st);
+        c->p.softening * c->p.softening, g, ux, uy, uz, gx, gy, gz, c->adj_tgt, stride, &c->st->err);
+    adj_fold_kernel<<<(int)std::min<int64_t>((ntgt + 255) / 256, 65536), 256, 0, s>>>(c->adj_tgt, ntgt, stride,
+                                                                                      c->st);
     const int64_t nblk = c->m - n + 1;
     adj_cells_kernel<<<(int)std::min<int64_t>((nblk + 255) / 256, 65536), 256, 0, s>>>(
         n, c->index, c->pblk, c->adj_tgt, c->mass64, c->mom64, c->adj_ab, c->st);

@@ -147,7 +147,7 @@ int nbx_destroy(nbx_context *c)
     hipSetDevice(c->device);
     hipDeviceSynchronize();
     void *bufs[] = {c->st, c->bbox_part, c->keys, c->skeys, c->kbuf, c->perm, c->vbuf, c->pos, c->hist,
-                    c->cell_level, c->parent, c->arrive, c->mass64, c->mom64, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->pblk, c->bpar, c->adj_tgt, c->adj_ab, c->adj_hot, c->counts,
+                    c->cell_level, c->parent, c->arrive, c->mass64, c->mom64, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->pblk, c->bpar, c->adj_tgt, c->adj_ab, c->counts,
                     c->acc_scratch, c->bodies4};
     for (void *b : bufs)
         if (b) hipFree(b);
@@ -372,17 +372,12 @@ int nbx_force_vjp(nbx_context *c, const float *ux, const float *uy, const float 
     hipSetDevice(c->device);
     if (!c->adj_tgt) {
         const size_t cnt = 4 * ((size_t)c->m_max + 1);
-        const int hot_cap = nbx::vjp_hot_capacity(c->n_max);
-        const size_t hot = 4 * (size_t)hot_cap * nbx::vjp_hot_copies();
-        if (dalloc(&c->adj_tgt, cnt) != hipSuccess || dalloc(&c->adj_ab, cnt) != hipSuccess ||
-            dalloc(&c->adj_hot, hot) != hipSuccess) {
+        if (dalloc(&c->adj_tgt, cnt * nbx::vjp_copies()) != hipSuccess || dalloc(&c->adj_ab, cnt) != hipSuccess) {
             if (c->adj_tgt) hipFree(c->adj_tgt);
             if (c->adj_ab) hipFree(c->adj_ab);
-            if (c->adj_hot) hipFree(c->adj_hot);
-            c->adj_tgt = c->adj_ab = c->adj_hot = nullptr;
+            c->adj_tgt = c->adj_ab = nullptr;
             return fail(c, NBX_ERR_HIP, "adjoint scratch allocation failed");
         }
-        c->adj_hot_cap = hot_cap;
     }
     hipStream_t s = (hipStream_t)stream;
     StageTimer t(c, s, elapsed_ms);

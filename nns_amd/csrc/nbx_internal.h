@@ -94,10 +94,8 @@ struct nbx_context {
     int32_t *pblk = nullptr;     // [m_max] parent block of each block (-1: root)
     int32_t *bpar = nullptr;     // [n_max] parent block of each body, by sorted position
     // adjoint scratch (allocated by the first nbx_force_vjp)
-    double *adj_tgt = nullptr;   // [4 (n_max + m_max)] gradient sums of bodies (sorted), then blocks
+    double *adj_tgt = nullptr;   // [copies][4 (m_max + 1)] gradient sums of bodies (sorted), then blocks
     double *adj_ab = nullptr;    // [4 m_max] per block: sum over ancestors of G/M and GM - G.X/M
-    double *adj_hot = nullptr;   // [copies][4 adj_hot_cap] spread sums of the root and top cells
-    int adj_hot_cap = 0;
     float layout_theta = -1.f;   // theta the frame blocks' thresholds were computed with
     int64_t *counts = nullptr;
     float *acc_scratch = nullptr;
@@ -210,7 +208,6 @@ void launch_batch_force(nbx_batch *b, hipStream_t s, int64_t *counts, int waves)
 // launchers (k_adjoint.hip)
 void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy, const float *uz, float *gx,
                 float *gy, float *gz, float *gm, float g);
-int vjp_hot_capacity(int n_max);  // blocks that can be the root or a top cell
-int vjp_hot_copies();
+int vjp_copies();  // copies of the adjoint's node sums (one per XCD)
 
 }  // namespace nbx
