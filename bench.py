@@ -101,6 +101,19 @@ def cpu_baseline(x, y, z, m, theta, softening, sample):
             "build_s": rec["build_s"], "force_s_extrapolated": round(force_full, 3)}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "force_kernel_binary_disk_4M_traffic.json")
+
+
+def force_traffic(n, theta):
+    """HBM bytes per launch of the force kernel on the headline workload, from the
+    counter runs recorded in profiles/ (FETCH_SIZE and WRITE_SIZE, each scaled by
+    what the counter reads for a 1 GiB device copy); None for other workloads."""
+    if n != 4194304 or theta != 0.5 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
 def _median(v):
     return float(np.median(np.asarray(v, np.float64)))
 
@@ -371,6 +384,16 @@ def main():
         accuracy = {"rms_rel_err_vs_fp64_direct": float(np.sqrt((rel ** 2).mean())),
                     "max_rel_err": float(rel.max()), "sample": int(len(tgt)),
                     "direct_sum_s": round(time.perf_counter() - t0, 2)}
+        # the same sample through the oracle's FP64 walk with the reference's l/D < theta:
+        # the error of the method itself, and the GPU's distance from it
+        t = O.KeyTree(xs, ys, zs, m)
+        ab, _ = t.accel(float(np.float32(params.theta)), float(np.float32(params.softening) ** 2),
+                        mac_mode=0, targets=tgt)
+        del t
+        rb = np.linalg.norm(ab - ad, axis=1) / np.linalg.norm(ad, axis=1)
+        rg = np.linalg.norm(agpu - ab, axis=1) / np.linalg.norm(ab, axis=1)
+        accuracy["fp64_tree_rms_rel_err_vs_fp64_direct"] = float(np.sqrt((rb ** 2).mean()))
+        accuracy["rms_rel_err_vs_fp64_tree"] = float(np.sqrt((rg ** 2).mean()))
 
     cpu = None
     if args.cpu_sample > 0:
@@ -400,7 +423,8 @@ def main():
         "interactions_per_step": int(i_alg),
         "wall_s": round(wall, 3),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                     "traffic": force_traffic(n, args.theta),
                      "kernel": "force_kernel", "flop_per_interaction": FLOP_PER_INTERACTION},
         "cpu_baseline": cpu,
         "accuracy": accuracy,
