@@ -197,9 +197,12 @@ int nbx_autotune(nbx_context *ctx, void *stream, int *waves_out);
  * u = dL/da (device arrays of n floats, input order), writes dL/dx, dL/dy, dL/dz and
  * dL/dm of every body (n floats each).  g is the gravity given to nbx_compute_forces.
  * Each body meets exactly the nodes of its forward interaction list; cell gradients
- * flow down to the bodies through the centre-of-mass sums.  Node sums use FP64
- * atomics, so the last bits may differ between runs.  The first call allocates
- * 288 (m_max + 1) bytes of scratch (eight copies of the node sums, one per XCD). */
+ * flow down to the bodies through the centre-of-mass sums (FP64).  The node terms
+ * are logged, sorted by node and summed; the order of records within a node
+ * follows the log, so the last bits may differ between runs.  The first call
+ * allocates the scratch: 64 (m_max + 1) bytes plus 40 bytes per log record, with
+ * 64 n_max + 4 n_max records (about 2.6 KB per body); if a call needs more
+ * records, the rest are added with atomics. */
 int nbx_force_vjp(nbx_context *ctx, const float *ux, const float *uy, const float *uz, float g, float *gx,
                   float *gy, float *gz, float *gm, void *stream, float *elapsed_ms);
 

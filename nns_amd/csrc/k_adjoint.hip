@@ -11,13 +11,18 @@
 // step writes one float4 per lane (rows padded to 65 entries, so the
 // transposed reads below are free of bank conflicts), and every kVjpBatch = 4
 // recorded steps the wave reads them back transposed (16 lanes per step, 4
-// entries each), folds the 16 partial sums with four DPP moves and adds one
-// FP64 atomic per component and node.  Bodies are nodes too: their sums go to
-// the body's own entry.  LDS stays under 7 KB per wave, so 16 waves fit a CU.
-// The sums have one copy per XCD (the wave's XCC_ID hardware register picks
-// it): a line that only one XCD updates stays in that XCD's L2.  The copy is a
-// placement choice, not a correctness one -- every copy takes device-scope
-// atomics -- and the flow-down pass adds the copies up first.
+// entries each) and folds the 16 partials with four DPP moves.  LDS stays under
+// 7 KB per wave, so 16 waves fit a CU.  Bodies are nodes too.
+//
+// The per-step sums do not go to the nodes by atomics (one FP64 atomic per
+// component and step is a memory-side transaction: at N = 262K they cost more
+// than the walk).  They are appended to a log {node, float4}: each wave takes
+// chunks of kChunk records with one atomicAdd and seals the unused tail of a
+// chunk with a sentinel node id.  The log is radix-sorted by node id (stable,
+// 3 passes for up to 2^24 - 1 nodes) and summed per node in FP64, in log order,
+// by threads that each walk 64 sorted records; only the node runs that cross a
+// thread's 64 records take an atomic.  A wave that finds the log full adds its
+// remaining sums with atomics directly.
 //
 // A cell's X and M are sums over its bodies, X = sum m_k x_k / M, so the node
 // gradients flow down: for every body k
@@ -31,7 +36,8 @@
 namespace nbx {
 
 constexpr int kVjpBatch = 4;
-constexpr int kXcdCopies = 8;
+constexpr int kChunk = 256;   // log records taken per atomicAdd
+constexpr int kSegRun = 64;   // sorted records summed per thread
 
 // Sum over each aligned group of 16 lanes (a DPP row), in every lane of the row.
 __device__ inline float seg16_sum(float v)
@@ -43,13 +49,6 @@ __device__ inline float seg16_sum(float v)
     return v;
 }
 
-__device__ inline int xcd_copy()
-{
-    unsigned v;
-    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
-    return (int)(v & (kXcdCopies - 1));
-}
-
 template <int W>
 __global__ void __launch_bounds__(64 * W) vjp_kernel(
     int n, int ngroups, const float4 *__restrict__ nodes4, const float4 *__restrict__ fnode,
@@ -57,7 +56,8 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
     const int32_t *__restrict__ perm, const DeviceState *st, float eps2, float G,
     const float *__restrict__ ux, const float *__restrict__ uy, const float *__restrict__ uz,
     float *__restrict__ gx, float *__restrict__ gy, float *__restrict__ gz, double *__restrict__ tgt,
-    int64_t copy_stride, int *__restrict__ err)
+    uint64_t *__restrict__ log_id, float4 *__restrict__ log_val, int64_t cap, uint64_t sentinel, DeviceState *dst,
+    int *__restrict__ err)
 {
     __shared__ uint4 stk[W][kForceStack];
     __shared__ float4 red[W][kVjpBatch][65];
@@ -70,7 +70,6 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
 
     const int first = g * 64;
     const int cnt_bodies = min(64, n - first);
-    double *const tcopy = tgt + copy_stride * xcd_copy();
 
     const int p = first + lane;
     const bool valid = lane < cnt_bodies;
@@ -83,6 +82,8 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
 
     float ox = 0.f, oy = 0.f, oz = 0.f;  // dL/dx of the lane's own body
     int nrec = 0;
+    int64_t cpos = 0, cend = 0;  // the wave's current log chunk [cpos, cend)
+    bool full = false;           // the log is full: add sums with atomics
     float4(*rw)[65] = red[wave];
     int *ri = rid[wave];
 
@@ -102,13 +103,31 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
         ay = seg16_sum(ay);
         az = seg16_sum(az);
         aw = seg16_sum(aw);
-        if (q_ == 0 && s_ < count) {
-            double *t = tcopy + 4 * (int64_t)ri[s_];
-            unsafeAtomicAdd(t + 0, (double)ax);
-            unsafeAtomicAdd(t + 1, (double)ay);
-            unsafeAtomicAdd(t + 2, (double)az);
-            unsafeAtomicAdd(t + 3, (double)aw);
+        if (!full && cend - cpos < count) {
+            if (lane < cend - cpos) log_id[cpos + lane] = sentinel;  // seal the old chunk (< 4 records)
+            int b = 0;
+            if (lane == 0) b = atomicAdd(&dst->log_next, kChunk);
+            b = __builtin_amdgcn_readfirstlane(b);
+            if ((int64_t)b + kChunk > cap) {
+                full = true;
+            } else {
+                cpos = b;
+                cend = (int64_t)b + kChunk;
+            }
         }
+        if (q_ == 0 && s_ < count) {
+            if (!full) {
+                log_id[cpos + s_] = (uint64_t)ri[s_];
+                log_val[cpos + s_] = make_float4(ax, ay, az, aw);
+            } else {
+                double *t = tgt + 4 * (int64_t)ri[s_];
+                unsafeAtomicAdd(t + 0, (double)ax);
+                unsafeAtomicAdd(t + 1, (double)ay);
+                unsafeAtomicAdd(t + 2, (double)az);
+                unsafeAtomicAdd(t + 3, (double)aw);
+            }
+        }
+        if (!full) cpos += count;
         __builtin_amdgcn_wave_barrier();
     };
 
@@ -203,6 +222,8 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
         }
     }
     if (nrec > 0) flush(nrec);
+    if (!full)
+        for (int64_t k = cpos + lane; k < cend; k += 64) log_id[k] = sentinel;  // seal the last chunk
     if (valid) {
         gx[i] = ox;
         gy[i] = oy;
@@ -217,15 +238,53 @@ __global__ void __launch_bounds__(256) clear_f64_kernel(double *__restrict__ p, 
 }
 
 // Per block b: A and B summed over b and its ancestors (walk up pblk).
-// Adds the per-XCD copies into copy 0 (count values each).
-__global__ void __launch_bounds__(256) adj_fold_kernel(double *__restrict__ tgt, int64_t count, int64_t copy_stride,
-                                                       const DeviceState *st)
+// Sort length: every granted chunk is written in full (records or sentinels).
+__global__ void log_count_kernel(DeviceState *st, int64_t cap)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const int64_t granted = (cap / kChunk) * kChunk;
+        st->log_count = (int)min((int64_t)st->log_next, granted);
+    }
+}
+
+// Per-node FP64 sums of the sorted log: thread t walks sorted records
+// [kSegRun t, kSegRun (t + 1)); a node run that starts before or ends after that
+// range is added with an atomic, a run inside it with a plain add of its own.
+__global__ void __launch_bounds__(256) log_reduce_kernel(const uint64_t *__restrict__ sk,
+                                                         const int32_t *__restrict__ sv,
+                                                         const float4 *__restrict__ val, const DeviceState *st,
+                                                         uint64_t sentinel, double *__restrict__ tgt)
 {
     if (latched(st)) return;
-    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < count; k += (int64_t)blockDim.x * gridDim.x) {
-        double v = tgt[k];
-        for (int c = 1; c < kXcdCopies; ++c) v += tgt[copy_stride * c + k];
-        tgt[k] = v;
+    const int64_t count = st->log_count;
+    const int64_t nthreads = (count + kSegRun - 1) / kSegRun;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nthreads;
+         t += (int64_t)blockDim.x * gridDim.x) {
+        const int64_t k0 = t * kSegRun, k1 = min(count, k0 + kSegRun);
+        uint64_t cur = sk[k0];
+        if (cur == sentinel) continue;
+        bool starts_here = k0 == 0 || sk[k0 - 1] != cur;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        for (int64_t k = k0;; ++k) {
+            const uint64_t id = k < k1 ? sk[k] : sentinel;
+            if (id != cur) {
+                // the run [.., k) of node cur ends here unless k == k1 and it continues past k1
+                const bool ends_here = k < k1 || k1 == count || sk[k1] != cur;
+                double *p = tgt + 4 * (int64_t)cur;
+                if (starts_here && ends_here) {
+                    p[0] += a0; p[1] += a1; p[2] += a2; p[3] += a3;
+                } else {
+                    unsafeAtomicAdd(p + 0, a0); unsafeAtomicAdd(p + 1, a1);
+                    unsafeAtomicAdd(p + 2, a2); unsafeAtomicAdd(p + 3, a3);
+                }
+                if (k >= k1 || id == sentinel) break;
+                cur = id;
+                starts_here = true;
+                a0 = a1 = a2 = a3 = 0.0;
+            }
+            const float4 v = val[sv[k]];
+            a0 += v.x; a1 += v.y; a2 += v.z; a3 += v.w;
+        }
     }
 }
 
@@ -287,7 +346,59 @@ __global__ void __launch_bounds__(256) adj_bodies_kernel(int n, const int32_t *_
     }
 }
 
-int vjp_copies() { return kXcdCopies; }
+// Radix passes over the node ids: ids are < m_max + 1 and the sentinel,
+// 2^(8 passes) - 1, must exceed them.
+static int sort_passes(int64_t m_max)
+{
+    int p = 1;
+    while ((int64_t(1) << (8 * p)) < m_max + 2) ++p;
+    return p;
+}
+
+__global__ void vjp_reset_kernel(DeviceState *st)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) st->log_next = 0;
+}
+
+template <class T> static bool dmalloc(T **p, int64_t count)
+{
+    return hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * (size_t)(count > 0 ? count : 1)) == hipSuccess;
+}
+
+bool alloc_vjp_scratch(nbx_context *c)
+{
+    const int64_t cnt = 4 * ((int64_t)c->m_max + 1);
+    const int64_t ngroups = ((int64_t)c->n_max + 63) / 64;
+    // about 60 records per body at theta = 0.5; a full log falls back to atomics
+    int64_t cap = 64 * (int64_t)c->n_max + (int64_t)kChunk * (ngroups + 1);
+    cap = std::min<int64_t>(cap, INT32_MAX - kSortTile);
+    const bool even = sort_passes(c->m_max) % 2 == 0;
+    const int64_t tiles = (cap + kSortTile - 1) / kSortTile;
+    bool ok = dmalloc(&c->adj_tgt, cnt) && dmalloc(&c->adj_ab, cnt) && dmalloc(&c->adj_log_id, cap) &&
+              dmalloc(&c->adj_log_val, cap) && dmalloc(&c->adj_sk, cap) && dmalloc(&c->adj_sv, cap) &&
+              dmalloc(&c->adj_tv, cap) && (!even || dmalloc(&c->adj_tk, cap)) &&
+              dmalloc(&c->adj_hist, tiles * kRadix + kRadix);
+    if (!ok) {
+        free_vjp_scratch(c);
+        return false;
+    }
+    c->adj_cap = cap;
+    return true;
+}
+
+void free_vjp_scratch(nbx_context *c)
+{
+    void *p[] = {c->adj_tgt, c->adj_ab, c->adj_log_id, c->adj_log_val, c->adj_sk, c->adj_sv, c->adj_tv, c->adj_tk,
+                 c->adj_hist};
+    for (void *q : p)
+        if (q) hipFree(q);
+    c->adj_tgt = c->adj_ab = nullptr;
+    c->adj_log_id = c->adj_sk = c->adj_tk = nullptr;
+    c->adj_log_val = nullptr;
+    c->adj_sv = c->adj_tv = nullptr;
+    c->adj_hist = nullptr;
+    c->adj_cap = 0;
+}
 
 void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy, const float *uz, float *gx,
                 float *gy, float *gz, float *gm, float g)
@@ -296,17 +407,23 @@ void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy,
     if (n <= 0) return;
     if (c->layout_theta != c->p.theta) launch_frame_layout(c, s);
     const int64_t ntgt = 4 * ((int64_t)n + (c->m - n + 1));  // bodies, then blocks
-    const int64_t stride = 4 * ((int64_t)c->m_max + 1);      // one copy
-    for (int k = 0; k < kXcdCopies; ++k)
-        clear_f64_kernel<<<(int)std::min<int64_t>((ntgt + 255) / 256, 65536), 256, 0, s>>>(c->adj_tgt + stride * k,
-                                                                                           ntgt);
+    clear_f64_kernel<<<(int)std::min<int64_t>((ntgt + 255) / 256, 65536), 256, 0, s>>>(c->adj_tgt, ntgt);
+    vjp_reset_kernel<<<1, 1, 0, s>>>(c->st);
+    const int passes = sort_passes(c->m_max);
+    const uint64_t sentinel = (uint64_t(1) << (8 * passes)) - 1;
     constexpr int W = 2;
     const int ngroups = (n + 63) / 64;
     vjp_kernel<W><<<(ngroups + W - 1) / W, 64 * W, 0, s>>>(
         n, ngroups, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->perm, c->st,
-        c->p.softening * c->p.softening, g, ux, uy, uz, gx, gy, gz, c->adj_tgt, stride, &c->st->err);
-    adj_fold_kernel<<<(int)std::min<int64_t>((ntgt + 255) / 256, 65536), 256, 0, s>>>(c->adj_tgt, ntgt, stride,
-                                                                                      c->st);
+        c->p.softening * c->p.softening, g, ux, uy, uz, gx, gy, gz, c->adj_tgt, c->adj_log_id, c->adj_log_val,
+        c->adj_cap, sentinel, c->st, &c->st->err);
+    log_count_kernel<<<1, 1, 0, s>>>(c->st, c->adj_cap);
+    uint64_t *tk = passes % 2 ? c->adj_log_id : c->adj_tk;  // the input keys are dead after the first pass
+    launch_radix_passes(c->adj_hist, s, c->adj_log_id, nullptr, (int)c->adj_cap, 0, passes, c->adj_sk, c->adj_sv, tk,
+                        c->adj_tv, nullptr, &c->st->log_count);
+    const int64_t runs = (c->adj_cap + kSegRun - 1) / kSegRun;
+    log_reduce_kernel<<<(int)std::min<int64_t>((runs + 255) / 256, 65536), 256, 0, s>>>(
+        c->adj_sk, c->adj_sv, c->adj_log_val, c->st, sentinel, c->adj_tgt);
     const int64_t nblk = c->m - n + 1;
     adj_cells_kernel<<<(int)std::min<int64_t>((nblk + 255) / 256, 65536), 256, 0, s>>>(
         n, c->index, c->pblk, c->adj_tgt, c->mass64, c->mom64, c->adj_ab, c->st);

@@ -28,9 +28,11 @@ __device__ inline uint64_t match_digit(uint32_t digit, uint64_t live)
 }
 
 __global__ void __launch_bounds__(kSortThreads) radix_upsweep(
-    const uint64_t *__restrict__ keys, int n, int shift, int ntiles, uint32_t *__restrict__ hist)
+    const uint64_t *__restrict__ keys, int n, int shift, int ntiles, uint32_t *__restrict__ hist,
+    const int *__restrict__ n_dev)
 {
     __shared__ uint32_t cnt[kSortThreads / 64][kRadix];
+    if (n_dev) n = min(n, *n_dev);  // keys beyond the device-side count take no part
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int k = tid; k < (kSortThreads / 64) * kRadix; k += kSortThreads) (&cnt[0][0])[k] = 0u;
     __syncthreads();
@@ -135,8 +137,9 @@ __global__ void __launch_bounds__(256) scan_add_kernel(uint32_t *__restrict__ da
 __global__ void __launch_bounds__(kSortThreads) radix_downsweep(
     const uint64_t *__restrict__ kin, const int32_t *__restrict__ vin, int n, int shift, int ntiles,
     const uint32_t *__restrict__ hist, const uint32_t *__restrict__ rowsum, uint64_t *__restrict__ kout,
-    int32_t *__restrict__ vout, int32_t *__restrict__ inv)
+    int32_t *__restrict__ vout, int32_t *__restrict__ inv, const int *__restrict__ n_dev)
 {
+    if (n_dev) n = min(n, *n_dev);
     constexpr int W = kSortThreads / 64;
     __shared__ uint32_t run[W][kRadix];
     __shared__ uint32_t dbase[kRadix];
@@ -193,7 +196,7 @@ __global__ void __launch_bounds__(kSortThreads) radix_downsweep(
 
 void launch_radix_passes(uint32_t *hist, hipStream_t s, const uint64_t *keys_in, const int32_t *vals_in, int n,
                          int first_shift, int npasses, uint64_t *keys_out, int32_t *vals_out, uint64_t *tmp_k,
-                         int32_t *tmp_v, int32_t *inv_out)
+                         int32_t *tmp_v, int32_t *inv_out, const int *n_dev)
 {
     if (n <= 0 || npasses <= 0) return;
     const int ntiles = (n + kSortTile - 1) / kSortTile;
@@ -207,10 +210,10 @@ void launch_radix_passes(uint32_t *hist, hipStream_t s, const uint64_t *keys_in,
         uint64_t *ko = to_out ? keys_out : tmp_k;
         int32_t *vo = to_out ? vals_out : tmp_v;
         uint32_t *rowsum = hist + (int64_t)ntiles * kRadix;  // kRadix totals after the rows
-        radix_upsweep<<<ntiles, kSortThreads, 0, s>>>(ki, n, shift, ntiles, hist);
+        radix_upsweep<<<ntiles, kSortThreads, 0, s>>>(ki, n, shift, ntiles, hist, n_dev);
         radix_rows_scan<<<kRadix, 256, 0, s>>>(hist, ntiles, rowsum);
         radix_downsweep<<<ntiles, kSortThreads, 0, s>>>(ki, vi, n, shift, ntiles, hist, rowsum, ko, vo,
-                                                        last ? inv_out : nullptr);
+                                                        last ? inv_out : nullptr, n_dev);
         ki = ko;
         vi = vo;
     }

@@ -147,11 +147,12 @@ int nbx_destroy(nbx_context *c)
     hipSetDevice(c->device);
     hipDeviceSynchronize();
     void *bufs[] = {c->st, c->bbox_part, c->keys, c->skeys, c->kbuf, c->perm, c->vbuf, c->pos, c->hist,
-                    c->cell_level, c->parent, c->arrive, c->mass64, c->mom64, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->pblk, c->bpar, c->adj_tgt, c->adj_ab, c->counts,
+                    c->cell_level, c->parent, c->arrive, c->mass64, c->mom64, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->pblk, c->bpar, c->counts,
                     c->acc_scratch, c->bodies4};
     for (void *b : bufs)
         if (b) hipFree(b);
     nbx::free_tree_scratch(c->ts);
+    nbx::free_vjp_scratch(c);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     for (auto &ev : c->stage_ev)
@@ -370,15 +371,7 @@ int nbx_force_vjp(nbx_context *c, const float *ux, const float *uy, const float 
     if (!ux || !uy || !uz || !gx || !gy || !gz || !gm) return fail(c, NBX_ERR_INVALID, "null array");
     if (!std::isfinite(g)) return fail(c, NBX_ERR_INVALID, "g must be finite");
     hipSetDevice(c->device);
-    if (!c->adj_tgt) {
-        const size_t cnt = 4 * ((size_t)c->m_max + 1);
-        if (dalloc(&c->adj_tgt, cnt * nbx::vjp_copies()) != hipSuccess || dalloc(&c->adj_ab, cnt) != hipSuccess) {
-            if (c->adj_tgt) hipFree(c->adj_tgt);
-            if (c->adj_ab) hipFree(c->adj_ab);
-            c->adj_tgt = c->adj_ab = nullptr;
-            return fail(c, NBX_ERR_HIP, "adjoint scratch allocation failed");
-        }
-    }
+    if (!c->adj_tgt && !nbx::alloc_vjp_scratch(c)) return fail(c, NBX_ERR_HIP, "adjoint scratch allocation failed");
     hipStream_t s = (hipStream_t)stream;
     StageTimer t(c, s, elapsed_ms);
     nbx::launch_vjp(c, s, ux, uy, uz, gx, gy, gz, gm, g);

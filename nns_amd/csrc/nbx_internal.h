@@ -49,6 +49,8 @@ struct DeviceState {
     int work2;          // next subtree to summarise
     int ntop;           // top cells (excluding roots)
     int topl[kLevels];  // top cells per level (lists in TreeScratch::top_list)
+    int log_next;       // adjoint: next free record of the node-term log
+    int log_count;      // adjoint: records written (sort length)
 };
 
 // Tree-construction scratch shared by the single-system and ensemble paths.
@@ -94,8 +96,14 @@ struct nbx_context {
     int32_t *pblk = nullptr;     // [m_max] parent block of each block (-1: root)
     int32_t *bpar = nullptr;     // [n_max] parent block of each body, by sorted position
     // adjoint scratch (allocated by the first nbx_force_vjp)
-    double *adj_tgt = nullptr;   // [copies][4 (m_max + 1)] gradient sums of bodies (sorted), then blocks
-    double *adj_ab = nullptr;    // [4 m_max] per block: sum over ancestors of G/M and GM - G.X/M
+    double *adj_tgt = nullptr;   // [4 (m_max + 1)] gradient sums of bodies (sorted), then blocks
+    double *adj_ab = nullptr;    // [4 (m_max + 1)] per block: sum over ancestors of G/M and GM - G.X/M
+    // node-term log: one record {node, float4} per child step with accepting lanes
+    int64_t adj_cap = 0;         // records
+    uint64_t *adj_log_id = nullptr, *adj_sk = nullptr, *adj_tk = nullptr;
+    float4 *adj_log_val = nullptr;
+    int32_t *adj_sv = nullptr, *adj_tv = nullptr;
+    uint32_t *adj_hist = nullptr;
     float layout_theta = -1.f;   // theta the frame blocks' thresholds were computed with
     int64_t *counts = nullptr;
     float *acc_scratch = nullptr;
@@ -180,9 +188,11 @@ void launch_radix_sort(nbx_context *c, hipStream_t s, const uint64_t *keys_in, i
 // exclusive scan of total <= 2^20 values; sums: scratch of 1024 values
 void launch_exclusive_scan_u32(uint32_t *data, int total, uint32_t *sums, hipStream_t s);
 // npasses passes of 8 bits from bit first_shift; vals_in == nullptr means 0, 1, 2, ...
+// n_dev (nullable): device-side count; only the first min(n, *n_dev) keys are sorted
+// (grids are sized for n).  tmp_k may alias keys_in when npasses is odd.
 void launch_radix_passes(uint32_t *hist, hipStream_t s, const uint64_t *keys_in, const int32_t *vals_in, int n,
                          int first_shift, int npasses, uint64_t *keys_out, int32_t *vals_out, uint64_t *tmp_k,
-                         int32_t *tmp_v, int32_t *inv_out);
+                         int32_t *tmp_v, int32_t *inv_out, const int *n_dev = nullptr);
 
 // launchers (k_tree.hip)
 void launch_build(nbx_context *c, hipStream_t s, int n, int m, int *child, int *index);
@@ -208,6 +218,8 @@ void launch_batch_force(nbx_batch *b, hipStream_t s, int64_t *counts, int waves)
 // launchers (k_adjoint.hip)
 void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy, const float *uz, float *gx,
                 float *gy, float *gz, float *gm, float g);
-int vjp_copies();  // copies of the adjoint's node sums (one per XCD)
+// Adjoint scratch of a context (log capacity from n_max; sort passes from m_max).
+bool alloc_vjp_scratch(nbx_context *c);
+void free_vjp_scratch(nbx_context *c);
 
 }  // namespace nbx
