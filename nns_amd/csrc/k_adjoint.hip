@@ -37,7 +37,6 @@ namespace nbx {
 
 constexpr int kVjpBatch = 4;
 constexpr int kChunk = 256;   // log records taken per atomicAdd
-constexpr int kSegRun = 64;   // sorted records summed per thread
 
 // Sum over each aligned group of 16 lanes (a DPP row), in every lane of the row.
 __device__ inline float seg16_sum(float v)
@@ -247,9 +246,12 @@ __global__ void log_count_kernel(DeviceState *st, int64_t cap)
     }
 }
 
-// Per-node FP64 sums of the sorted log: thread t walks sorted records
-// [kSegRun t, kSegRun (t + 1)); a node run that starts before or ends after that
-// range is added with an atomic, a run inside it with a plain add of its own.
+// Per-node sums of the sorted log, one wave per 64 sorted records (one per
+// lane, gathered from the log in one instruction): an inclusive scan over the
+// lanes holding the same node (six shuffle steps), then the last lane of each
+// run adds it to the node.  A run that continues into the previous or the next
+// 64 records is added with an FP64 atomic; a run inside the 64 with a plain add
+// (no other wave holds that node).
 __global__ void __launch_bounds__(256) log_reduce_kernel(const uint64_t *__restrict__ sk,
                                                          const int32_t *__restrict__ sv,
                                                          const float4 *__restrict__ val, const DeviceState *st,
@@ -257,33 +259,37 @@ __global__ void __launch_bounds__(256) log_reduce_kernel(const uint64_t *__restr
 {
     if (latched(st)) return;
     const int64_t count = st->log_count;
-    const int64_t nthreads = (count + kSegRun - 1) / kSegRun;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nthreads;
-         t += (int64_t)blockDim.x * gridDim.x) {
-        const int64_t k0 = t * kSegRun, k1 = min(count, k0 + kSegRun);
-        uint64_t cur = sk[k0];
-        if (cur == sentinel) continue;
-        bool starts_here = k0 == 0 || sk[k0 - 1] != cur;
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        for (int64_t k = k0;; ++k) {
-            const uint64_t id = k < k1 ? sk[k] : sentinel;
-            if (id != cur) {
-                // the run [.., k) of node cur ends here unless k == k1 and it continues past k1
-                const bool ends_here = k < k1 || k1 == count || sk[k1] != cur;
-                double *p = tgt + 4 * (int64_t)cur;
-                if (starts_here && ends_here) {
-                    p[0] += a0; p[1] += a1; p[2] += a2; p[3] += a3;
-                } else {
-                    unsafeAtomicAdd(p + 0, a0); unsafeAtomicAdd(p + 1, a1);
-                    unsafeAtomicAdd(p + 2, a2); unsafeAtomicAdd(p + 3, a3);
-                }
-                if (k >= k1 || id == sentinel) break;
-                cur = id;
-                starts_here = true;
-                a0 = a1 = a2 = a3 = 0.0;
+    const int lane = threadIdx.x & 63;
+    const int64_t ntiles = (count + 63) / 64;
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < ntiles; w += wstride) {
+        const int64_t k0 = w * 64, k = k0 + lane;
+        const uint64_t id = k < count ? sk[k] : sentinel;
+        const bool live = id != sentinel;
+        float4 v = live ? val[sv[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const int id32 = live ? (int)id : -1;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int oid = __shfl_up(id32, o);
+            const float tx = __shfl_up(v.x, o), ty = __shfl_up(v.y, o), tz = __shfl_up(v.z, o),
+                        tw = __shfl_up(v.w, o);
+            if (lane >= o && oid == id32) {
+                v.x += tx; v.y += ty; v.z += tz; v.w += tw;
             }
-            const float4 v = val[sv[k]];
-            a0 += v.x; a1 += v.y; a2 += v.z; a3 += v.w;
+        }
+        const int next = __shfl_down(id32, 1);
+        const bool last_of_run = live && (lane == 63 || next != id32);
+        const int lane0_id = __shfl(id32, 0);
+        const bool from_prev = k0 > 0 && lane0_id == id32 && sk[k0 - 1] == id;
+        const bool into_next = lane == 63 && k + 1 < count && sk[k + 1] == id;
+        if (last_of_run) {
+            double *p = tgt + 4 * (int64_t)id;
+            if (from_prev || into_next) {
+                unsafeAtomicAdd(p + 0, (double)v.x); unsafeAtomicAdd(p + 1, (double)v.y);
+                unsafeAtomicAdd(p + 2, (double)v.z); unsafeAtomicAdd(p + 3, (double)v.w);
+            } else {
+                p[0] += v.x; p[1] += v.y; p[2] += v.z; p[3] += v.w;
+            }
         }
     }
 }
@@ -421,8 +427,8 @@ void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy,
     uint64_t *tk = passes % 2 ? c->adj_log_id : c->adj_tk;  // the input keys are dead after the first pass
     launch_radix_passes(c->adj_hist, s, c->adj_log_id, nullptr, (int)c->adj_cap, 0, passes, c->adj_sk, c->adj_sv, tk,
                         c->adj_tv, nullptr, &c->st->log_count);
-    const int64_t runs = (c->adj_cap + kSegRun - 1) / kSegRun;
-    log_reduce_kernel<<<(int)std::min<int64_t>((runs + 255) / 256, 65536), 256, 0, s>>>(
+    const int64_t tiles = (c->adj_cap + 63) / 64;  // one wave per 64 sorted records
+    log_reduce_kernel<<<(int)std::min<int64_t>((tiles + 3) / 4, 65536), 256, 0, s>>>(
         c->adj_sk, c->adj_sv, c->adj_log_val, c->st, sentinel, c->adj_tgt);
     const int64_t nblk = c->m - n + 1;
     adj_cells_kernel<<<(int)std::min<int64_t>((nblk + 255) / 256, 65536), 256, 0, s>>>(
