@@ -10,7 +10,9 @@
   taken step by step from the last state to the first.  States are
   checkpointed every ``every`` steps; the backward re-runs one segment from its
   checkpoint with one context per step, so that every step's tree is still
-  held when its vector-Jacobian product is taken (no second build).
+  held when its vector-Jacobian product is taken (no second build).  The last
+  step of a re-run segment only builds its tree: the state after it is the
+  next checkpoint (or the final state), already known.
 
 No counterpart in the reference; the force and update arithmetic is that of
 the plain method.
@@ -33,12 +35,17 @@ class Slots:
         z = np.zeros(self.n, np.float32)
         self.bh = [BarnesHut(z, z, z, z, z, z, z, params=self.params, device=device) for _ in range(count)]
 
-    def accel(self, k: int, x, y, z, m):
-        """Builds slot k's tree for (x, y, z, m) and returns its accelerations (new tensors)."""
+    def tree(self, k: int, x, y, z, m):
+        """Builds slot k's tree for (x, y, z, m)."""
         bh, n = self.bh[k], self.n
         for key, t in (("x", x), ("y", y), ("z", z), ("mass", m)):
             bh.arr[key][:n].copy_(t)
         bh.rebuild()
+
+    def accel(self, k: int, x, y, z, m):
+        """Builds slot k's tree for (x, y, z, m) and returns its accelerations (new tensors)."""
+        bh, n = self.bh[k], self.n
+        self.tree(k, x, y, z, m)
         bh.forces()
         return bh.arr["ax"][:n].clone(), bh.arr["ay"][:n].clone(), bh.arr["az"][:n].clone()
 
@@ -67,7 +74,7 @@ class BarnesHutAccel(torch.autograd.Function):
         slots = ctx.slots
         zero = torch.zeros_like(x)
         u = [g.contiguous() if g is not None else zero for g in (gax, gay, gaz)]
-        slots.accel(0, x, y, z, m)  # the tree of these positions
+        slots.tree(0, x, y, z, m)  # the tree of these positions
         gx, gy, gz, gm = slots.vjp(0, *u)
         return gx, gy, gz, gm, None
 
@@ -75,10 +82,10 @@ class BarnesHutAccel(torch.autograd.Function):
 def _step(slots: Slots, k: int, s, m, dt: float):
     x, y, z, vx, vy, vz = s
     ax, ay, az = slots.accel(k, x, y, z, m)
-    vx = vx + dt * ax
-    vy = vy + dt * ay
-    vz = vz + dt * az
-    return (x + dt * vx, y + dt * vy, z + dt * vz, vx, vy, vz)
+    vx = torch.add(vx, ax, alpha=dt)
+    vy = torch.add(vy, ay, alpha=dt)
+    vz = torch.add(vz, az, alpha=dt)
+    return (torch.add(x, vx, alpha=dt), torch.add(y, vy, alpha=dt), torch.add(z, vz, alpha=dt), vx, vy, vz)
 
 
 class Rollout(torch.autograd.Function):
@@ -109,20 +116,22 @@ class Rollout(torch.autograd.Function):
         for seg in reversed(range(nseg)):
             k0 = seg * every
             k1 = min(steps, k0 + every)
-            # re-run the segment: slot j keeps the tree of state k0 + j
+            # re-run the segment: slot j keeps the tree of state k0 + j; the last
+            # state's tree is all its step needs (the state after it is known)
             s = ctx.ckpt[seg]
-            for k in range(k0, k1):
+            for k in range(k0, k1 - 1):
                 s = _step(slots, k - k0, s, m, dt)
+            slots.tree(k1 - 1 - k0, s[0], s[1], s[2], m)
             for k in reversed(range(k0, k1)):
                 # x_{k+1} = x_k + dt v_{k+1}, v_{k+1} = v_k + dt a(x_k)
-                lvx = lvx + dt * lx
-                lvy = lvy + dt * ly
-                lvz = lvz + dt * lz
+                lvx = torch.add(lvx, lx, alpha=dt)
+                lvy = torch.add(lvy, ly, alpha=dt)
+                lvz = torch.add(lvz, lz, alpha=dt)
                 ax, ay, az, am = slots.vjp(k - k0, lvx, lvy, lvz)
-                lx = lx + dt * ax
-                ly = ly + dt * ay
-                lz = lz + dt * az
-                lm = lm + dt * am
+                lx = torch.add(lx, ax, alpha=dt)
+                ly = torch.add(ly, ay, alpha=dt)
+                lz = torch.add(lz, az, alpha=dt)
+                lm = torch.add(lm, am, alpha=dt)
         return lx, ly, lz, lvx, lvy, lvz, lm, None, None, None, None
 
 
