@@ -52,7 +52,8 @@ template <int W>
 __global__ void __launch_bounds__(64 * W) vjp_kernel(
     int n, int ngroups, const float4 *__restrict__ nodes4, const float4 *__restrict__ fnode,
     const int2 *__restrict__ fmeta, const float4 *__restrict__ froot4, const int2 *__restrict__ frootm,
-    const int32_t *__restrict__ perm, const DeviceState *st, float eps2, float G,
+    const int32_t *__restrict__ perm, const int32_t *__restrict__ gorder, const DeviceState *st, float eps2,
+    float G,
     const float *__restrict__ ux, const float *__restrict__ uy, const float *__restrict__ uz,
     float *__restrict__ gx, float *__restrict__ gy, float *__restrict__ gz, double *__restrict__ tgt,
     uint64_t *__restrict__ log_id, float4 *__restrict__ log_val, int64_t cap, uint64_t sentinel, DeviceState *dst,
@@ -70,8 +71,8 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
     const int first = g * 64;
     const int cnt_bodies = min(64, n - first);
 
-    const int p = first + lane;
     const bool valid = lane < cnt_bodies;
+    const int p = valid ? gorder[first + lane] : 0;  // the lane's body, by sorted position
     const float4 me = valid ? nodes4[p] : make_float4(0.f, 0.f, 0.f, 0.f);
     const int i = valid ? perm[p] : 0;
     const float vx = valid ? G * ux[i] : 0.f;
@@ -420,7 +421,7 @@ void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy,
     constexpr int W = 2;
     const int ngroups = (n + 63) / 64;
     vjp_kernel<W><<<(ngroups + W - 1) / W, 64 * W, 0, s>>>(
-        n, ngroups, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->perm, c->st,
+        n, ngroups, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->perm, c->gorder, c->st,
         c->p.softening * c->p.softening, g, ux, uy, uz, gx, gy, gz, c->adj_tgt, c->adj_log_id, c->adj_log_val,
         c->adj_cap, sentinel, c->st, &c->st->err);
     log_count_kernel<<<1, 1, 0, s>>>(c->st, c->adj_cap);

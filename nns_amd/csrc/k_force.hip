@@ -31,8 +31,9 @@
 // Blocks are remapped so that the blocks one XCD receives (dispatch is
 // round-robin over the 8 XCDs) walk consecutive groups and share its L2.
 //
-// Single system: group g is sorted positions [64g, 64g+64) and the walk starts
-// from froot[0].  Ensemble: group g comes from a table {first, count, system};
+// Single system: group g is bodies [64g, 64g+64) of the group order (gorder:
+// sorted positions ordered by coarse Hilbert key) and the walk starts from
+// froot[0].  Ensemble: group g comes from a table {first, count, system};
 // the walk starts at froot[system] (thresholds of the system's own theta and
 // extent are in the descriptors).
 #include "nbx_device.h"
@@ -43,7 +44,8 @@ template <bool KAHAN, bool COUNT, bool SOFT, bool BATCH, int W>
 __global__ void __launch_bounds__(64 * W) force_kernel(
     int n, int ngroups, const float4 *__restrict__ nodes4, const float4 *__restrict__ fnode,
     const int2 *__restrict__ fmeta, const float4 *__restrict__ froot4, const int2 *__restrict__ frootm,
-    const int32_t *__restrict__ perm, const DeviceState *st, float eps2, float G, float *__restrict__ ax,
+    const int32_t *__restrict__ perm, const int32_t *__restrict__ gorder, const DeviceState *st, float eps2,
+    float G, float *__restrict__ ax,
     float *__restrict__ ay, float *__restrict__ az, int64_t *__restrict__ counts, int *__restrict__ err,
     const int4 *__restrict__ groups)
 {
@@ -65,8 +67,8 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
         cnt_bodies = min(64, n - first);
         sys = 0;
     }
-    const int p = first + lane;
     const bool valid = lane < cnt_bodies;
+    const int p = valid ? gorder[first + lane] : 0;  // the lane's body, by sorted position
     const float4 me = valid ? nodes4[p] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint64_t live = __ballot(valid);
 
@@ -184,7 +186,7 @@ struct ForceArgs {
     int n, ngroups;
     const float4 *nodes4, *fnode, *froot4;
     const int2 *fmeta, *frootm;
-    const int32_t *perm;
+    const int32_t *perm, *gorder;
     const DeviceState *st;
     float eps2, G;
     float *ax, *ay, *az;
@@ -198,7 +200,7 @@ static void launch_one(const ForceArgs &a, hipStream_t s)
 {
     const int nblk = (a.ngroups + W - 1) / W;
     force_kernel<KAHAN, COUNT, SOFT, BATCH, W><<<nblk, 64 * W, 0, s>>>(
-        a.n, a.ngroups, a.nodes4, a.fnode, a.fmeta, a.froot4, a.frootm, a.perm, a.st, a.eps2, a.G, a.ax, a.ay, a.az,
+        a.n, a.ngroups, a.nodes4, a.fnode, a.fmeta, a.froot4, a.frootm, a.perm, a.gorder, a.st, a.eps2, a.G, a.ax, a.ay, a.az,
         a.counts, a.err, a.groups);
 }
 
@@ -232,7 +234,7 @@ static ForceArgs single_args(nbx_context *c, float *ax, float *ay, float *az, fl
     a.n = c->n;
     a.ngroups = (c->n + 63) / 64;
     a.nodes4 = c->nodes4; a.fnode = c->fnode; a.fmeta = c->fmeta; a.froot4 = c->froot4; a.frootm = c->frootm;
-    a.perm = c->perm; a.st = c->st;
+    a.perm = c->perm; a.gorder = c->gorder; a.st = c->st;
     a.eps2 = c->p.softening * c->p.softening;
     a.G = g;
     a.ax = ax; a.ay = ay; a.az = az; a.counts = counts;
@@ -260,7 +262,7 @@ void launch_batch_force(nbx_batch *b, hipStream_t s, int64_t *counts, int waves)
     a.n = b->N;
     a.ngroups = b->ngroups;
     a.nodes4 = b->nodes4; a.fnode = b->fnode; a.fmeta = b->fmeta; a.froot4 = b->froot4; a.frootm = b->frootm;
-    a.perm = b->perm; a.st = b->st;
+    a.perm = b->perm; a.gorder = b->gorder; a.st = b->st;
     a.eps2 = b->p.softening * b->p.softening;
     a.G = b->p.gravity;
     a.ax = counts ? b->acc_scratch : b->ax;

@@ -627,7 +627,29 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
     }
 }
 
+// Group order of the force walk.  The walk gives each wave 64 consecutive
+// bodies of this order.  Key order has jumps (consecutive Morton keys can lie in
+// cells far apart), so bodies are ordered by the top kGroupLevels Hilbert digits
+// of their key instead, stably over key order (ties keep key order); the
+// ensemble puts the system index above them.  The order changes which bodies
+// share a wave, never a body's interaction list.
+__global__ void __launch_bounds__(256) group_keys_kernel(int n, const uint64_t *__restrict__ skeys,
+                                                         const int32_t *__restrict__ perm,
+                                                         const int32_t *__restrict__ sys_of,
+                                                         uint64_t *__restrict__ gkey, const DeviceState *st)
+{
+    if (latched(st)) return;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += blockDim.x * gridDim.x) {
+        uint64_t h = hilbert_top(skeys[k] & ~(1ull << 63), kGroupLevels);
+        if (sys_of) h |= (uint64_t)sys_of[perm[k]] << (3 * kGroupLevels);
+        gkey[k] = h;
+    }
+}
+
 // ------------------------------------------------------------------ launchers
+// Radix passes of the group-order keys (Hilbert digits, then the system index).
+static int group_passes(int sys_bits) { return (3 * kGroupLevels + sys_bits + kRadixBits - 1) / kRadixBits; }
+
 static inline int capped_grid(int64_t work, int cap = 65536)
 {
     int64_t g = (work + 255) / 256;
@@ -709,6 +731,9 @@ void launch_layout(nbx_context *c, hipStream_t s, int *sorted)
     const int n = c->n;
     layout_bodies_kernel<<<capped_grid(n), 256, 0, s>>>(n, c->perm, c->bodies4, c->nodes4, sorted, c->st);
     launch_frame_layout(c, s);
+    group_keys_kernel<<<capped_grid(n), 256, 0, s>>>(n, c->skeys, c->perm, nullptr, c->gkey, c->st);
+    launch_radix_passes(c->hist, s, c->gkey, nullptr, n, 0, group_passes(0), c->gkey2, c->gorder, c->kbuf, c->vbuf,
+                        nullptr);
 }
 
 void launch_frame_layout(nbx_context *c, hipStream_t s)
@@ -742,6 +767,11 @@ void launch_batch_layout(nbx_batch *b, hipStream_t s)
     frame_layout_kernel<true><<<capped_grid(8 * (int64_t)(b->m - b->N)), 256, 0, s>>>(
         b->N, b->S, b->index, b->child, b->pos, b->cell_level, b->nodes4, b->bodies4, b->fnode, b->fmeta, b->froot4,
         b->frootm, b->pblk, b->bpar, 0.f, si, b->st);
+    int sys_bits = 0;
+    while ((1 << sys_bits) < b->S) ++sys_bits;
+    group_keys_kernel<<<capped_grid(b->N), 256, 0, s>>>(b->N, b->skeys, b->perm, b->sys_of, b->gkey, b->st);
+    launch_radix_passes(b->hist, s, b->gkey, nullptr, b->N, 0, group_passes(sys_bits), b->gkey2, b->gorder, b->kbuf,
+                        b->vbuf, nullptr);
 }
 
 }  // namespace nbx

@@ -128,6 +128,8 @@ int nbx_create(const nbx_params *p, int n_max, int m_max, int device, nbx_contex
               dalloc(&c->froot4, 1) == hipSuccess && dalloc(&c->frootm, 1) == hipSuccess &&
               dalloc(&c->pblk, m) == hipSuccess && dalloc(&c->bpar, n) == hipSuccess && dalloc(&c->counts, n) == hipSuccess &&
               dalloc(&c->acc_scratch, 3 * n) == hipSuccess && dalloc(&c->bodies4, n) == hipSuccess &&
+              dalloc(&c->gkey, n) == hipSuccess && dalloc(&c->gkey2, n) == hipSuccess &&
+              dalloc(&c->gorder, n) == hipSuccess &&
               hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
     for (int k = 0; ok && k < 8; ++k) ok = hipEventCreate(&c->stage_ev[k]) == hipSuccess;
     if (ok) ok = nbx::alloc_tree_scratch(c->ts, n_max, 1, (n_max + nbx::kSubtreeMax - 1) / nbx::kSubtreeMax);
@@ -148,7 +150,7 @@ int nbx_destroy(nbx_context *c)
     hipDeviceSynchronize();
     void *bufs[] = {c->st, c->bbox_part, c->keys, c->skeys, c->kbuf, c->perm, c->vbuf, c->pos, c->hist,
                     c->cell_level, c->parent, c->arrive, c->mass64, c->mom64, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->pblk, c->bpar, c->counts,
-                    c->acc_scratch, c->bodies4};
+                    c->acc_scratch, c->bodies4, c->gkey, c->gkey2, c->gorder};
     for (void *b : bufs)
         if (b) hipFree(b);
     nbx::free_tree_scratch(c->ts);

@@ -64,6 +64,21 @@ static __device__ inline uint64_t hilbert_of(uint64_t lebesgue)
     return h;
 }
 
+// The first `levels` Hilbert digits of a Lebesgue key (the top 3*levels bits of
+// hilbert_of): the mapping runs from the top level down, so stopping early
+// leaves the coarse digits exact.
+static __device__ inline uint64_t hilbert_top(uint64_t lebesgue, int levels)
+{
+    uint64_t h = 0;
+    int dir = 0;
+    for (int lvl = kKeyLevels; lvl > kKeyLevels - levels; --lvl) {
+        int cell = (int)((lebesgue >> ((lvl - 1) * 3)) & 7u);
+        h = (h << 3) + c_hil[dir][cell];
+        dir = c_dir[dir][cell];
+    }
+    return h;
+}
+
 // ---------------------------------------------------------------- tree
 static __device__ inline int digit_at(uint64_t key, int level) { return (int)((key >> (60 - 3 * level)) & 7u); }
 

@@ -108,6 +108,8 @@ struct nbx_context {
     int64_t *counts = nullptr;
     float *acc_scratch = nullptr;
     float4 *bodies4 = nullptr;  // (x, y, z, m) of every body in input order
+    uint64_t *gkey = nullptr, *gkey2 = nullptr;  // group-order sort keys
+    int32_t *gorder = nullptr;  // [n] sorted position of the j-th body of the force groups
     nbx::TreeScratch ts;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t stage_ev[8] = {};
@@ -154,6 +156,8 @@ struct nbx_batch {
     int64_t *counts = nullptr;
     float *acc_scratch = nullptr;
     float4 *bodies4 = nullptr;
+    uint64_t *gkey = nullptr, *gkey2 = nullptr;
+    int32_t *gorder = nullptr;
     int32_t *samp_off = nullptr;  // [S+1] first top-table sample of each system
     nbx::TreeScratch ts;
     hipEvent_t stage_ev[8] = {};
@@ -200,6 +204,8 @@ void launch_summarize(nbx_context *c, hipStream_t s);
 void launch_layout(nbx_context *c, hipStream_t s, int *sorted);
 // Frame blocks of the force walk for the current theta (part of launch_layout).
 void launch_frame_layout(nbx_context *c, hipStream_t s);
+// Hilbert digits kept for the force groups' order (top levels of the key).
+constexpr int kGroupLevels = 10;
 
 // launchers (k_force.hip)
 void launch_force(nbx_context *c, hipStream_t s, float *ax, float *ay, float *az, float g, int64_t *counts);
