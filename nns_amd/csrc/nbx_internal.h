@@ -13,11 +13,11 @@ namespace nbx {
 constexpr int kKeyLevels = 21;
 constexpr double kMaskCoord = 2097151.0;
 
-// Radix sort: 8-bit digits, 256 threads x 16 items per tile.
+// Radix sort: 8-bit digits, 256 threads x 8 items per tile.
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
+constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
 
 // Force traversal: one body per lane, 64-body groups, depth-first stack per wave.
