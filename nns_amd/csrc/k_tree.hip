@@ -249,105 +249,8 @@ __device__ inline int block_sum(int v, int *red)
 //                they are equal), then publishes the first cell with a release store
 //                (unlock).
 // No lane waits inside an iteration on another lane, so a wave whose lanes
-// contend for one slot always makes progress.  The subtree's exact cell count
-// is known before the insertion, so when it fits (kLdsCells) the child slots
-// live in LDS during the insertion -- locks, compare-and-swaps and descents are
-// LDS operations -- and the finished subtree is copied to `child` with coalesced
-// stores; larger subtrees (long chains of nearly equal keys) are built in place
-// with workgroup-scope atomics on global memory.
-constexpr int kLdsCells = 1536;
-
-// Slot s of cell `node` (global id), in LDS (cells base .. base + kLdsCells) or in `child`.
-template <bool LDS>
-struct SlotRef {
-    int *slots;   // LDS: the workgroup's slot array; global: child
-    int base;
-    __device__ int *at(int node, int s) const
-    {
-        return LDS ? slots + 8 * (node - base) + s : slots + 8 * (int64_t)node + s;
-    }
-};
-
-template <bool LDS>
-__device__ void insert_subtree(const SlotRef<LDS> ref, int n, int a, int count, int L0, int base, int ks,
-                               const uint64_t *__restrict__ skeys, const int32_t *__restrict__ perm,
-                               const uint64_t *__restrict__ keys, int32_t *__restrict__ cell_level, int *s_next,
-                               DeviceState *st)
-{
-    const int tid = threadIdx.x;
-    const int root = base;
-    const int rounds = (count + blockDim.x - 1) / blockDim.x;
-    for (int r = 0; r < rounds; ++r) {
-        const int off = r + tid * rounds;
-        bool done = off >= count;
-        const int p = a + off;
-        const int body = done ? 0 : perm[p];
-        const uint64_t key = done ? 0ull : skeys[p];
-        int node = root, L = L0;
-        while (__ballot(!done)) {
-            if (done) continue;
-            if (L == kKeyLevels) {
-                bool placed = false, full = true;
-                for (int s = 0; s < 8 && !placed; ++s) {
-                    int ch = ld_slot_wg(ref.at(node, s));
-                    if (ch == -1) {
-                        full = false;
-                        if (cas_slot_wg(ref.at(node, s), -1, body) == -1) placed = true;
-                    }
-                }
-                if (placed) {
-                    done = true;
-                } else if (full) {
-                    atomicOr(&st->err, kErrBucket);
-                    done = true;
-                }
-                continue;
-            }
-            int *slot = ref.at(node, digit_at(key, L));
-            const int ch = ld_slot_wg(slot);
-            if (ch >= n) {
-                node = ch;
-                ++L;
-            } else if (ch == -1) {
-                if (cas_slot_wg(slot, -1, body) == -1) done = true;
-            } else if (ch >= 0) {
-                if (cas_slot_wg(slot, ch, -2) == ch) {
-                    const uint64_t key2 = keys[ch];
-                    int ld = L + 1;
-                    while (ld < kKeyLevels && digit_at(key, ld) == digit_at(key2, ld)) ++ld;
-                    const int need = ld - L;  // cells at levels L+1 .. ld
-                    const int old = atomicAdd(s_next, need);
-                    if (old + need > ks) {
-                        atomicOr(&st->err, kErrCapacity);
-                        st_slot_wg(slot, ch);
-                        done = true;
-                        continue;
-                    }
-                    const int first = base + old;
-                    for (int j = 0; j < need; ++j) {
-                        const int c = first + j;
-                        const int lvl = L + 1 + j;
-                        for (int s = 0; s < 8; ++s) *ref.at(c, s) = -1;
-                        cell_level[c] = lvl;
-                        if (lvl < ld) {
-                            *ref.at(c, digit_at(key, lvl)) = c + 1;
-                        } else if (ld < kKeyLevels) {
-                            *ref.at(c, digit_at(key2, ld)) = ch;
-                            *ref.at(c, digit_at(key, ld)) = body;
-                        } else {
-                            *ref.at(c, 0) = ch;  // bucket; index order restored in the summary
-                            *ref.at(c, 1) = body;
-                        }
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                    st_slot_wg(slot, first);  // publish + unlock
-                    done = true;
-                }
-            }
-        }
-    }
-}
-
+// contend for one slot always makes progress.  All atomics and fences are at
+// workgroup scope: only this workgroup touches the subtree.
 __global__ void __launch_bounds__(256) subtree_build_kernel(
     int n, int m, const uint64_t *__restrict__ skeys, const int32_t *__restrict__ perm,
     const uint64_t *__restrict__ keys, const uint8_t *__restrict__ lcp, int *child, int *index,
@@ -355,7 +258,6 @@ __global__ void __launch_bounds__(256) subtree_build_kernel(
 {
     __shared__ int s_job, s_base, s_next, s_bad;
     __shared__ int red[4];
-    __shared__ int s_slots[8 * kLdsCells];
     if (latched(st)) return;
     const int tid = threadIdx.x;
     const int nsub = __hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -375,7 +277,6 @@ __global__ void __launch_bounds__(256) subtree_build_kernel(
         }
         const int ks = block_sum(contrib, red);
         const int Lmax = block_max(deep, red);
-        const bool in_lds = ks <= kLdsCells;
         if (tid == 0) {
             const int base = atomicAdd(index, ks);
             s_bad = (int64_t)base + ks > (int64_t)m;
@@ -383,9 +284,9 @@ __global__ void __launch_bounds__(256) subtree_build_kernel(
             s_base = base;
             s_next = 1;
             if (!s_bad) {
-                int *r0 = in_lds ? s_slots : child + 8 * (int64_t)base;
-                for (int d = 0; d < 8; ++d) r0[d] = -1;
+                for (int d = 0; d < 8; ++d) child[8 * (int64_t)base + d] = -1;
                 cell_level[base] = L0;
+                child[slot] = base;
                 subcells[job] = make_int4(base, ks, L0, Lmax);
             } else {
                 subcells[job] = make_int4(0, 0, L0, L0 - 1);
@@ -393,24 +294,82 @@ __global__ void __launch_bounds__(256) subtree_build_kernel(
         }
         __syncthreads();
         if (s_bad) continue;
-        const int base = s_base;
-        if (in_lds) {
-            insert_subtree<true>(SlotRef<true>{s_slots, base}, n, a, b - a, L0, base, ks, skeys, perm, keys,
-                                 cell_level, &s_next, st);
-            __syncthreads();
-            int4 *dst = reinterpret_cast<int4 *>(child + 8 * (int64_t)base);
-            const int4 *src = reinterpret_cast<const int4 *>(s_slots);
-            for (int k = tid; k < 2 * ks; k += blockDim.x) dst[k] = src[k];
-        } else {
-            insert_subtree<false>(SlotRef<false>{child, base}, n, a, b - a, L0, base, ks, skeys, perm, keys,
-                                  cell_level, &s_next, st);
+        const int base = s_base, root = base;
+        const int count = b - a;
+        const int rounds = (count + blockDim.x - 1) / blockDim.x;
+        for (int r = 0; r < rounds; ++r) {
+            const int off = r + tid * rounds;
+            bool done = off >= count;
+            const int p = a + off;
+            const int body = done ? 0 : perm[p];
+            const uint64_t key = done ? 0ull : skeys[p];
+            int node = root, L = L0;
+            while (__ballot(!done)) {
+                if (done) continue;
+                if (L == kKeyLevels) {
+                    bool placed = false, full = true;
+                    for (int s = 0; s < 8 && !placed; ++s) {
+                        int ch = ld_slot_wg(&child[8 * (int64_t)node + s]);
+                        if (ch == -1) {
+                            full = false;
+                            if (cas_slot_wg(&child[8 * (int64_t)node + s], -1, body) == -1) placed = true;
+                        }
+                    }
+                    if (placed) {
+                        done = true;
+                    } else if (full) {
+                        atomicOr(&st->err, kErrBucket);
+                        done = true;
+                    }
+                    continue;
+                }
+                const int64_t addr = 8 * (int64_t)node + digit_at(key, L);
+                const int ch = ld_slot_wg(&child[addr]);
+                if (ch >= n) {
+                    node = ch;
+                    ++L;
+                } else if (ch == -1) {
+                    if (cas_slot_wg(&child[addr], -1, body) == -1) done = true;
+                } else if (ch >= 0) {
+                    if (cas_slot_wg(&child[addr], ch, -2) == ch) {
+                        const uint64_t key2 = keys[ch];
+                        int ld = L + 1;
+                        while (ld < kKeyLevels && digit_at(key, ld) == digit_at(key2, ld)) ++ld;
+                        const int need = ld - L;  // cells at levels L+1 .. ld
+                        const int old = atomicAdd(&s_next, need);
+                        if (old + need > ks) {
+                            atomicOr(&st->err, kErrCapacity);
+                            st_slot_wg(&child[addr], ch);
+                            done = true;
+                            continue;
+                        }
+                        const int first = base + old;
+                        const int4 neg = make_int4(-1, -1, -1, -1);
+                        for (int j = 0; j < need; ++j) {
+                            const int c = first + j;
+                            const int lvl = L + 1 + j;
+                            int4 *c4 = reinterpret_cast<int4 *>(child + 8 * (int64_t)c);
+                            c4[0] = neg;
+                            c4[1] = neg;
+                            cell_level[c] = lvl;
+                            if (lvl < ld) {
+                                child[8 * (int64_t)c + digit_at(key, lvl)] = c + 1;
+                            } else if (ld < kKeyLevels) {
+                                child[8 * (int64_t)c + digit_at(key2, ld)] = ch;
+                                child[8 * (int64_t)c + digit_at(key, ld)] = body;
+                            } else {
+                                child[8 * (int64_t)c + 0] = ch;  // bucket; index order restored in the summary
+                                child[8 * (int64_t)c + 1] = body;
+                            }
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                        st_slot_wg(&child[addr], first);  // publish + unlock
+                        done = true;
+                    }
+                }
+            }
         }
         __syncthreads();
-        if (tid == 0) {
-            // publish the subtree in its parent's slot once its cells are in place
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            child[slot] = base;
-        }
     }
 }
 
