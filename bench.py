@@ -205,7 +205,8 @@ def run_batched(args):
             "batched_stage_ms": {k: round(_median([t[j] for t in stage_b]), 4) for j, k in enumerate(names)},
             "plain_ms_per_step": round(ms_p, 4), "plain_stage_ms": stage_p, "plain_interactions_per_step": i_p,
             "plain_g_interactions_per_s": round(ips_p, 2),
-            "batched_over_plain": round(ips_b / ips_p, 3)}
+            "batched_over_plain": round(ips_b / ips_p, 3),
+            "batched_over_plain_bodies_per_s": round(ms_p / ms_b, 3)}
 
 
 def run_adjoint(args):

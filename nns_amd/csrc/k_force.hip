@@ -76,7 +76,6 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
     float cx = 0.f, cy = 0.f, cz = 0.f;  // compensations
     float px = 0.f, py = 0.f, pz = 0.f;  // partials of the current tile
     int64_t cnt = 0;
-    int tile = 0;
 
     // One child step: every lane of `mask` meets child q with descriptor
     // {tag, crit}; returns the lanes that open it.  The monopole is evaluated
@@ -116,10 +115,18 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
             sp = 1;
         }
     }
-    while (sp > 0) {
+    // Frames run in tiles of kTile; the partials are folded after each tile.
+    // The frame loop has one exit, so the accumulators keep their registers
+    // from frame to frame (no copies between iterations).
+    constexpr int kTile = KAHAN ? 4 : 0x7fffffff;
+    bool overflow = false;
+    do {
+      for (int t = 0; t < kTile; ++t) {
+        if (sp <= 0) break;
         // a frame pops one entry and pushes at most 8
         if (sp > kForceStack - 7) {
-            if (lane == 0) atomicOr(err, kErrStack);
+            overflow = true;
+            sp = 0;
             break;
         }
         sp = __builtin_amdgcn_readfirstlane(sp - 1);
@@ -159,18 +166,14 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
                 sp = __builtin_amdgcn_readfirstlane(sp + 1);
             }
         }
-        if (KAHAN) {  // fold the partials every 4 frames (about 20 children)
-            tile = __builtin_amdgcn_readfirstlane(tile + 1);
-            if (tile == 4) {
-                kahan_add(sx, cx, px); kahan_add(sy, cy, py); kahan_add(sz, cz, pz);
-                px = py = pz = 0.f;
-                tile = 0;
-            }
+      }
+        if (KAHAN) {  // fold the partials of the tile (about 20 children)
+            kahan_add(sx, cx, px); kahan_add(sy, cy, py); kahan_add(sz, cz, pz);
+            px = py = pz = 0.f;
         }
-    }
-    if (KAHAN) {
-        kahan_add(sx, cx, px); kahan_add(sy, cy, py); kahan_add(sz, cz, pz);
-    } else {
+    } while (sp > 0);
+    if (overflow && lane == 0) atomicOr(err, kErrStack);
+    if (!KAHAN) {
         sx = px; sy = py; sz = pz;
     }
     if (valid) {
