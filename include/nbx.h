@@ -77,6 +77,10 @@ typedef struct nbx_params {
     int kahan;        /* 1: compensated (Kahan) accumulation of the accelerations       */
     int block_size;   /* threads per block of the streaming stages (0 = 256)            */
     int force_waves;  /* waves per block of the force kernel (0 = 4)                    */
+    int build_locked; /* 1: subtrees built by lock-based insertion (compare-and-swap on
+                         child slots, Kernels.cu:138-241); 0: from the shared-digit
+                         counts of neighbouring sorted keys, one store per slot.  Both
+                         give the same cells and child slots.                          */
 } nbx_params;
 
 /* Fills p with the engine defaults: theta 0.5, softening 1e-3, G 1, dt 1e-3, d 1, Kahan on. */

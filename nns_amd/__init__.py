@@ -53,7 +53,7 @@ class NbxError(RuntimeError):
 class Params(C.Structure):
     _fields_ = [("theta", C.c_float), ("softening", C.c_float), ("gravity", C.c_float),
                 ("timestep", C.c_float), ("dampening", C.c_float), ("kahan", C.c_int),
-                ("block_size", C.c_int), ("force_waves", C.c_int)]
+                ("block_size", C.c_int), ("force_waves", C.c_int), ("build_locked", C.c_int)]
 
 
 class Arrays(C.Structure):

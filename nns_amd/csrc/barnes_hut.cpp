@@ -62,6 +62,7 @@ BarnesHut::BarnesHut(const SimulationParameters &p) : parameters(p)
     np.timestep = p.timestep;
     np.dampening = p.dampening;
     np.kahan = p.kahan ? 1 : 0;
+    np.build_locked = p.buildLocked ? 1 : 0;
     check(nbx_create(&np, numParticles, numNodes, 0, &ctx), "nbx_create");
 }
 

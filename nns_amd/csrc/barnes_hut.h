@@ -21,6 +21,7 @@ struct SimulationParameters {  // include/Constants.h:4-22 plus the engine's kno
     float theta = 0.5f;
     float softening = 1e-3f;
     bool kahan = true;
+    bool buildLocked = false;           // subtree build by lock-based insertion
     std::string model = "binary_disk";  // star_disk | plummer | binary_disk
     unsigned long long seed = 1;
     bool copyToHost = false;            // the reference copies x, v back every step

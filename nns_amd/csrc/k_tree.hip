@@ -14,10 +14,13 @@
 //      from the sorted keys by binary search (one thread per sample of
 //      kSubtreeMax bodies) and wired with their child ranges;
 //   2. every child range of a top cell with 2..kSubtreeMax bodies is a subtree
-//      built by ONE workgroup with the lock-based insertion (compare-and-swap
-//      on child slots, -2 = locked) at workgroup scope; its exact cell count
-//      comes from the common-prefix lengths of neighbouring keys, so one
-//      atomicAdd reserves its contiguous block of cell ids;
+//      built by ONE wave from the shared-digit counts of neighbouring keys:
+//      cell ids from a prefix sum, parents from previous-smaller-value queries,
+//      one plain store per child slot (subtree_build_kernel).  Its exact cell
+//      count is known first, so one atomicAdd reserves its block of cell ids.
+//      The lock-based insertion (compare-and-swap on child slots, -2 = locked,
+//      at workgroup scope; subtree_insert_kernel) builds the same cells when
+//      nbx_params.build_locked is set;
 //   3. the summary runs level by level: subtrees inside their workgroup, then
 //      the top cells in one block.  Every cell is summed by one thread over its
 //      children in slot order, in FP64: results do not depend on scheduling.
@@ -239,7 +242,8 @@ __device__ inline int block_sum(int v, int *red)
     return s;
 }
 
-// Subtree build: persistent workgroups take subtrees from the queue.  Inside a
+// Subtree build by insertion (nbx_params.build_locked = 1): persistent
+// workgroups take subtrees from the queue.  Inside a
 // subtree, the bodies are inserted by lock-based descent along their key digits
 // (one body per thread per round, bodies strided over the range so that the
 // lanes of a wave start in different branches):
@@ -251,7 +255,7 @@ __device__ inline int block_sum(int v, int *red)
 // No lane waits inside an iteration on another lane, so a wave whose lanes
 // contend for one slot always makes progress.  All atomics and fences are at
 // workgroup scope: only this workgroup touches the subtree.
-__global__ void __launch_bounds__(256) subtree_build_kernel(
+__global__ void __launch_bounds__(256) subtree_insert_kernel(
     int n, int m, const uint64_t *__restrict__ skeys, const int32_t *__restrict__ perm,
     const uint64_t *__restrict__ keys, const uint8_t *__restrict__ lcp, int *child, int *index,
     int32_t *__restrict__ cell_level, int4 *__restrict__ sub, int4 *__restrict__ subcells, DeviceState *st)
@@ -368,6 +372,154 @@ __global__ void __launch_bounds__(256) subtree_build_kernel(
                     }
                 }
             }
+        }
+        __syncthreads();
+    }
+}
+
+// Subtree build, without locks or atomics on child slots.  One wave builds one
+// subtree [a, b) of sorted bodies whose root is at level L0 (a child range of
+// a top cell, 2..kSubtreeMax bodies).  With l[i] the number of key digits that
+// local bodies i and i+1 share (l[cnt-1] = L0 - 1 at the subtree's edge) and
+// P_i = l[i-1] (P_0 = L0 - 1), the cells of the subtree are the runs of two or
+// more bodies that share L digits, L = L0..21, and body i starts exactly the
+// cells of levels P_i < L <= l[i].  Numbering cells by first body, then level,
+//     id(i, L) = base + cs[i] + (L - P_i - 1),   cs = exclusive sum of max(0, l[i] - P_i),
+// gives every cell its id from one prefix sum (base from one atomicAdd).
+// The parent of cell (i, L), L > L0, is (i, L-1) when L-1 > P_i; otherwise the
+// level-P_i cell that holds bodies i-1 and i, which starts at
+//     s_i = 1 + the last j < i-1 with l[j] < P_i   (0 if none),
+// a previous-smaller-value query answered by binary lifting over a table of
+// range minima of l in LDS.  The deepest cell of body i is (i, l[i]) when
+// l[i] > P_i, else the same (s_i, P_i).  A child takes the slot of its key
+// digit at the parent's level; in a level-21 bucket (equal keys) the bodies
+// take slots 0, 1, ... in sorted order, which is ascending index order.
+// Each slot thus has exactly one writer: the wave sets the slots of its cells
+// to -1, waits for those stores, and every cell and body stores itself into
+// its parent.  Every id and slot is range-checked before its store.
+constexpr int kLiftLevels = 10;  // 2^kLiftLevels >= kSubtreeMax
+constexpr int kBuildBlocks = 4096;
+
+__device__ inline int wave_exclusive_sum(int v, int *total)
+{
+    const int lane = threadIdx.x & 63;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    *total = __shfl(x, 63);
+    return x - v;
+}
+
+__global__ void __launch_bounds__(64) subtree_build_kernel(
+    int m, const uint64_t *__restrict__ skeys, const int32_t *__restrict__ perm, const uint8_t *__restrict__ lcp,
+    int *__restrict__ child, int *index, int32_t *__restrict__ cell_level, const int4 *__restrict__ sub,
+    int sub_max, int4 *__restrict__ subcells, DeviceState *st)
+{
+    __shared__ uint8_t s_min[kLiftLevels][kSubtreeMax];  // s_min[j][x] = min of l over [x, x + 2^j)
+    __shared__ uint16_t s_cs[kSubtreeMax];                 // cs[i] (ids relative to base)
+    if (latched(st)) return;
+    const int lane = threadIdx.x;
+    const int nsub = min(__hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), sub_max);
+    for (;;) {
+        int job = 0;
+        if (lane == 0) job = atomicAdd(&st->work, 1);
+        job = __builtin_amdgcn_readfirstlane(__shfl(job, 0));
+        if (job >= nsub) break;
+        const int4 jb = sub[job];
+        const int a = jb.x, cnt = jb.y - jb.x, L0 = jb.z, slot = jb.w;
+        if (cnt < 2 || cnt > kSubtreeMax || L0 < 1 || L0 > kKeyLevels) {
+            if (lane == 0) {
+                atomicOr(&st->err, kErrCapacity);
+                subcells[job] = make_int4(0, 0, L0, L0 - 1);
+            }
+            continue;
+        }
+        // l[] into LDS; the edge position takes L0 - 1
+        for (int i = lane; i < cnt; i += 64) s_min[0][i] = (uint8_t)(i < cnt - 1 ? (int)lcp[a + i] : L0 - 1);
+        __syncthreads();
+        // each lane owns a contiguous chunk of bodies [i0, i1)
+        const int chunk = (cnt + 63) >> 6;
+        const int i0 = min(cnt, lane * chunk), i1 = min(cnt, i0 + chunk);
+        int mine = 0, deep = L0;
+        for (int i = i0; i < i1; ++i) {
+            const int P = i == 0 ? L0 - 1 : (int)s_min[0][i - 1];
+            const int l = s_min[0][i];
+            mine += max(0, l - P);
+            deep = max(deep, l);
+        }
+        int ks;
+        int run = wave_exclusive_sum(mine, &ks);
+        for (int o = 32; o > 0; o >>= 1) deep = max(deep, __shfl_xor(deep, o));
+        for (int i = i0; i < i1; ++i) {
+            const int P = i == 0 ? L0 - 1 : (int)s_min[0][i - 1];
+            s_cs[i] = (uint16_t)run;
+            run += max(0, (int)s_min[0][i] - P);
+        }
+        int base = 0;
+        if (lane == 0) base = atomicAdd(index, ks);
+        base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
+        if (ks < 1 || ks > 65535 || (int64_t)base + ks > (int64_t)m) {
+            if (lane == 0) {
+                atomicOr(&st->err, kErrCapacity);
+                subcells[job] = make_int4(0, 0, L0, L0 - 1);
+            }
+            __syncthreads();
+            continue;
+        }
+        // range-minimum table over the cnt - 1 shared-digit counts
+        for (int j = 1; j < kLiftLevels && (1 << j) <= cnt - 1; ++j) {
+            __syncthreads();
+            const int h = 1 << (j - 1);
+            for (int x = lane; x + (1 << j) <= cnt - 1; x += 64)
+                s_min[j][x] = (uint8_t)min((int)s_min[j - 1][x], (int)s_min[j - 1][x + h]);
+        }
+        // empty slots and cell levels
+        int *cslots = child + 8 * (int64_t)base;
+        for (int k = lane; k < 8 * ks; k += 64) cslots[k] = -1;
+        for (int i = i0; i < i1; ++i) {
+            const int P = i == 0 ? L0 - 1 : (int)s_min[0][i - 1];
+            const int l = s_min[0][i];
+            for (int L = P + 1; L <= l; ++L) cell_level[base + (int)s_cs[i] + (L - P - 1)] = L;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __syncthreads();
+        // every cell (but the root) and every body into its parent's slot
+        bool bad = false;
+        for (int i = i0; i < i1; ++i) {
+            const int P = i == 0 ? L0 - 1 : (int)s_min[0][i - 1];
+            const int l = s_min[0][i];
+            const uint64_t key = skeys[a + i];
+            const int own = (int)s_cs[i] - P - 1;  // id(i, L) - base = own + L
+            // start of the level-P run through i (only used for i >= 1)
+            int s = 0;
+            if (i >= 1) {
+                int p = i - 1;
+                for (int j = kLiftLevels - 1; j >= 0; --j) {
+                    const int w = 1 << j;
+                    if (p - w >= 0 && (int)s_min[j][p - w] >= P) p -= w;
+                }
+                s = p;
+            }
+            const int Ps = s == 0 ? L0 - 1 : (int)s_min[0][s - 1];
+            const int up = (int)s_cs[s] - Ps - 1 + P;  // id(s, P) - base
+            for (int L = max(P + 1, L0 + 1); L <= l; ++L) {
+                const int par = L - 1 > P ? own + L - 1 : up;
+                const int d = digit_at(key, L - 1);
+                if (par < 0 || par >= ks) { bad = true; continue; }
+                child[8 * ((int64_t)base + par) + d] = base + own + L;
+            }
+            const int D = max(P, l);
+            const int par = l > P ? own + l : up;
+            const int d = D < kKeyLevels ? digit_at(key, D) : (l > P ? 0 : i - s);
+            if (par < 0 || par >= ks || d >= 8) { bad = true; continue; }
+            child[8 * ((int64_t)base + par) + d] = perm[a + i];
+        }
+        if (__ballot(bad) && lane == 0) atomicOr(&st->err, kErrBucket);
+        if (lane == 0) {
+            child[slot] = base;
+            subcells[job] = make_int4(base, ks, L0, deep);
         }
         __syncthreads();
     }
@@ -661,7 +813,7 @@ constexpr int kPersistentBlocks = 2048;  // 256 CUs x 8 workgroups
 
 static void build_tree(Segs g, int nsamples, int base0, int m, TreeScratch &ts, DeviceState *st, hipStream_t s,
                        int n, const uint64_t *skeys, const int32_t *perm, const uint64_t *keys, int *child,
-                       int *index, int32_t *cell_level)
+                       int *index, int32_t *cell_level, bool locked)
 {
     lcp_kernel<<<capped_grid(n), 256, 0, s>>>(n, skeys, ts.lcp, st);
     top_cells_kernel<<<(nsamples + 255) / 256, 256, 0, s>>>(g, nsamples, skeys, ts.top_flag, ts.top_id, ts.top_a,
@@ -671,8 +823,12 @@ static void build_tree(Segs g, int nsamples, int base0, int m, TreeScratch &ts, 
     top_wire_kernel<<<(wire + 255) / 256, 256, 0, s>>>(g, nsamples, skeys, perm, child, cell_level, ts.top_flag,
                                                         ts.top_id, ts.top_a, ts.top_b, ts.sub, ts.sub_max, ts.top_list,
                                                         base0, m, index, st);
-    subtree_build_kernel<<<kPersistentBlocks, 256, 0, s>>>(n, m, skeys, perm, keys, ts.lcp, child, index, cell_level,
-                                                           ts.sub, ts.subcells, st);
+    if (locked)
+        subtree_insert_kernel<<<kPersistentBlocks, 256, 0, s>>>(n, m, skeys, perm, keys, ts.lcp, child, index,
+                                                                cell_level, ts.sub, ts.subcells, st);
+    else
+        subtree_build_kernel<<<kBuildBlocks, 64, 0, s>>>(m, skeys, perm, ts.lcp, child, index, cell_level, ts.sub,
+                                                         ts.sub_max, ts.subcells, st);
 }
 
 static void summarise_tree(Segs g, int nsamples, TreeScratch &ts, DeviceState *st, hipStream_t s,
@@ -714,8 +870,8 @@ void launch_build(nbx_context *c, hipStream_t s, int n, int m, int *child, int *
 {
     if (n <= 0) return;
     Segs g{nullptr, nullptr, 1, n, 0};
-    build_tree(g, samples_of(n), n, m, c->ts, c->st, s, n, c->skeys, c->perm, c->keys, child, index,
-               c->cell_level);
+    build_tree(g, samples_of(n), n, m, c->ts, c->st, s, n, c->skeys, c->perm, c->keys, child, index, c->cell_level,
+               c->p.build_locked != 0);
 }
 
 void launch_summarize(nbx_context *c, hipStream_t s)
@@ -749,7 +905,7 @@ void launch_batch_build(nbx_batch *b, hipStream_t s)
 {
     Segs g{b->off, b->samp_off, b->S, b->N, 1};
     build_tree(g, b->ts.samples_max, b->N + b->S, b->m, b->ts, b->st, s, b->N, b->skeys, b->perm, b->keys, b->child,
-               b->index, b->cell_level);
+               b->index, b->cell_level, b->p.build_locked != 0);
 }
 
 void launch_batch_summarize(nbx_batch *b, hipStream_t s)

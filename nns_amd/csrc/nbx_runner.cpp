@@ -3,7 +3,7 @@
 // per-stage [TIME] lines per step and the per-stage totals at the end.
 //
 //   nbx_runner [-n particles] [-i iterations] [-v] [--model binary_disk|plummer|star_disk]
-//              [--theta T] [--softening E] [--seed S] [--copy]
+//              [--theta T] [--softening E] [--seed S] [--copy] [--locked-build]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -33,9 +33,10 @@ int main(int argc, char **argv)
         else if (a == "--softening") p.softening = (float)std::atof(next("--softening"));
         else if (a == "--seed") p.seed = std::strtoull(next("--seed"), nullptr, 10);
         else if (a == "--copy") p.copyToHost = true;
+        else if (a == "--locked-build") p.buildLocked = true;
         else if (a == "-h" || a == "--help") {
             std::printf("nbx_runner [-n particles] [-i iterations] [-v] [--model binary_disk|plummer|star_disk]\n"
-                        "           [--theta T] [--softening E] [--seed S] [--copy]\n");
+                        "           [--theta T] [--softening E] [--seed S] [--copy] [--locked-build]\n");
             return 0;
         } else {
             std::fprintf(stderr, "unknown option %s\n", a.c_str());

@@ -82,30 +82,33 @@ def check_tree(bh, t):
     """Cells, children, FP64 summaries and counts against the oracle's key tree."""
     n = bh.n
     g = gpu_cells(bh)
-    # oracle cell 0 is the root; GPU cells are [n, index)
+    # oracle cell 0 is the root; GPU cells are [n, index).  A cell of the key
+    # tree is named by (level, first sorted position): match the two sets.
     assert g["nc"] == t.ncell - 1
-    okey = {(int(t.level[c]), int(t.start[c])): c for c in range(1, t.ncell)}
-    gid = {}
-    for k in range(g["nc"]):
-        key = (int(g["level"][k]), int(g["start"][k]))
-        assert key in okey, key
-        gid[n + k] = okey[key]
-    oc = np.array([gid[n + k] for k in range(g["nc"])], np.int64)
+    gk = g["level"].astype(np.int64) << 32 | g["start"].astype(np.int64)
+    ok = t.level[1:].astype(np.int64) << 32 | t.start[1:].astype(np.int64)
+    gi, oi = np.argsort(gk, kind="stable"), np.argsort(ok, kind="stable")
+    assert np.array_equal(gk[gi], ok[oi])
+    assert np.unique(gk).size == gk.size
+    oc = np.empty(g["nc"], np.int64)
+    oc[gi] = oi + 1                                        # oracle cell of each GPU cell
     assert np.array_equal(g["count"], t.count[oc])
     assert np.array_equal(g["mass"], t.mass[oc])          # FP64, bit for bit
     assert np.array_equal(g["com"], t.com[oc])
     assert np.array_equal(g["fx"], t.c32[oc, 0]) and np.array_equal(g["fm"], t.c32[oc, 3])
     assert g["root"][0] == t.mass[0] and np.array_equal(g["root"][1:], t.com[0])
 
-    def enc(v):  # GPU child value -> oracle encoding
-        return v if v < n else -2 - gid[v]
-    for s in range(8):
-        v = int(g["child"][0, s])
-        assert (v if v < 0 else enc(v)) == t.child[0, s]
-    ch = g["child"][n:]
-    for k in range(g["nc"]):
-        got = [v if v < 0 else enc(int(v)) for v in ch[k]]
-        assert got == list(t.child[oc[k]]), (k, got, t.child[oc[k]])
+    def enc(v):  # GPU child values -> oracle encoding (-1 empty, body id, -2 - cell)
+        v = v.astype(np.int64)
+        assert ((v >= -1) & (v < n + g["nc"])).all()
+        out = v.copy()
+        cell = v >= n
+        out[cell] = -2 - oc[v[cell] - n]
+        return out
+    assert np.array_equal(enc(g["child"][0]), t.child[0])
+    ch = enc(g["child"][n:])
+    bad = np.nonzero((ch != t.child[oc]).any(axis=1))[0]
+    assert bad.size == 0, (bad[:4], ch[bad[:4]], t.child[oc[bad[:4]]])
 
 
 def check_forces(bh, t, oracle, eps2, tol_rms=1e-5, tol_max=2e-3):
@@ -149,6 +152,47 @@ def test_pipeline_matches_oracle(oracle, kind, n):
     assert np.array_equal(bh.host("sorted"), t.perm)
     check_tree(bh, t)
     check_forces(bh, t, oracle, float(np.float32(np.float32(params.softening) ** 2)))
+
+
+@pytest.mark.parametrize("kind,n", [("star_disk", 32768), ("coincident", 5000), ("plummer", 130)])
+def test_locked_build_matches_oracle(oracle, kind, n):
+    """The lock-based subtree insertion (build_locked = 1) builds the same tree."""
+    _torch()
+    print(f"case locked {kind} n={n}", flush=True)
+    x, y, z, vx, vy, vz, m = make_case(kind, n)
+    params = nns_amd.default_params(theta=float(THETA), build_locked=1)
+    m_nodes = 24 * n + 12000 if kind == "coincident" else None
+    bh = run_to_forces(x, y, z, vx, vy, vz, m, params, m_nodes=m_nodes)
+    t = oracle.KeyTree(x, y, z, m, cap=(m_nodes - n) if m_nodes else 0)
+    assert np.array_equal(bh.host("sorted"), t.perm)
+    check_tree(bh, t)
+    check_forces(bh, t, oracle, float(np.float32(np.float32(params.softening) ** 2)))
+
+
+def test_full_size_binary_disk(oracle):
+    """The headline workload at its full size (binary + disk, N = 4,194,304,
+    theta 0.5, eps 1e-3, Kahan on): keys, sorted order, every cell, child slot
+    and FP64 summary bit for bit; interaction counts and accelerations on a
+    seeded sample of 4096 bodies against the oracle's walk of the same tree."""
+    _torch()
+    n = 4194304
+    print(f"case binary_disk n={n}", flush=True)
+    x, y, z, vx, vy, vz, m = nns_amd.initial_conditions("binary_disk", n, 1)
+    params = nns_amd.default_params(theta=float(THETA))
+    bh = run_to_forces(x, y, z, vx, vy, vz, m, params)
+    t = oracle.KeyTree(x, y, z, m)
+    assert t.err == 0
+    assert np.array_equal(bh.ctx.copy_buffer(nns_amd.BUF_SORTED_KEYS, n, np.uint64), t.skeys)
+    assert np.array_equal(bh.host("sorted"), t.perm)
+    check_tree(bh, t)
+    eps2 = float(np.float32(np.float32(params.softening) ** 2))
+    tg = np.sort(np.random.default_rng(9).choice(n, 4096, replace=False)).astype(np.int32)
+    a_orc, cnt_orc = t.accel(params.theta, eps2, mac_mode=1, targets=tg)
+    ag = np.stack([bh.host("ax")[tg], bh.host("ay")[tg], bh.host("az")[tg]], 1).astype(np.float64)
+    G = params.gravity
+    rel = np.linalg.norm(ag - G * a_orc, axis=1) / (G * np.linalg.norm(a_orc, axis=1))
+    assert np.sqrt((rel ** 2).mean()) <= 1e-5 and rel.max() <= 2e-3, (np.sqrt((rel ** 2).mean()), rel.max())
+    assert np.array_equal(bh.count_interactions().cpu().numpy()[tg], cnt_orc)
 
 
 def test_unsoftened_and_no_kahan(oracle):
