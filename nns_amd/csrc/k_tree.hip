@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(256) top_wire_kernel(
     Segs g, int nsamples, const uint64_t *__restrict__ skeys, const int32_t *__restrict__ perm, int *child,
     int32_t *__restrict__ cell_level, const uint32_t *__restrict__ flag, const uint32_t *__restrict__ id,
     const int32_t *__restrict__ ta, const int32_t *__restrict__ tb, int4 *__restrict__ sub, int sub_max,
-    int32_t *__restrict__ top_list, int base0, int m, int *index, DeviceState *st)
+    int32_t *__restrict__ top_list, int base0, int m, int *index, int *__restrict__ start, DeviceState *st)
 {
     if (latched(st)) return;
     const int total = nsamples * kLevels;
@@ -163,6 +163,7 @@ __global__ void __launch_bounds__(256) top_wire_kernel(
         s = seg_of_sample(g, k);
         if (c >= m) return;  // capacity error latched by thread 0
         cell_level[c] = L;
+        start[c] = a;
         top_list[L * nsamples + atomicAdd(&st->topl[L], 1)] = c;
         if (L == kKeyLevels) {  // more than kSubtreeMax bodies on one key
             atomicOr(&st->err, kErrBucket);
@@ -175,6 +176,7 @@ __global__ void __launch_bounds__(256) top_wire_kernel(
         L = 0;
         a = seg_lo(g, s);
         b = seg_hi(g, s);
+        if (g.batch) start[c] = a;  // the single system's root is node 0, not a cell
     } else {
         return;
     }
@@ -414,19 +416,16 @@ __device__ inline int wave_exclusive_sum(int v, int *total)
 
 __global__ void __launch_bounds__(64) subtree_build_kernel(
     int m, const uint64_t *__restrict__ skeys, const int32_t *__restrict__ perm, const uint8_t *__restrict__ lcp,
-    int *__restrict__ child, int *index, int32_t *__restrict__ cell_level, const int4 *__restrict__ sub,
-    int sub_max, int4 *__restrict__ subcells, DeviceState *st)
+    int *__restrict__ child, int *index, int32_t *__restrict__ cell_level, int *__restrict__ start,
+    const int4 *__restrict__ sub, int sub_max, int4 *__restrict__ subcells, DeviceState *st)
 {
+    constexpr int C = kSubtreeMax / 64;                   // bodies per lane at most
     __shared__ uint8_t s_min[kLiftLevels][kSubtreeMax];  // s_min[j][x] = min of l over [x, x + 2^j)
     __shared__ uint16_t s_cs[kSubtreeMax];                 // cs[i] (ids relative to base)
     if (latched(st)) return;
     const int lane = threadIdx.x;
     const int nsub = min(__hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), sub_max);
-    for (;;) {
-        int job = 0;
-        if (lane == 0) job = atomicAdd(&st->work, 1);
-        job = __builtin_amdgcn_readfirstlane(__shfl(job, 0));
-        if (job >= nsub) break;
+    for (int job = blockIdx.x; job < nsub; job += gridDim.x) {
         const int4 jb = sub[job];
         const int a = jb.x, cnt = jb.y - jb.x, L0 = jb.z, slot = jb.w;
         if (cnt < 2 || cnt > kSubtreeMax || L0 < 1 || L0 > kKeyLevels) {
@@ -436,14 +435,33 @@ __global__ void __launch_bounds__(64) subtree_build_kernel(
             }
             continue;
         }
-        // l[] into LDS; the edge position takes L0 - 1
-        for (int i = lane; i < cnt; i += 64) s_min[0][i] = (uint8_t)(i < cnt - 1 ? (int)lcp[a + i] : L0 - 1);
-        __syncthreads();
-        // each lane owns a contiguous chunk of bodies [i0, i1)
+        // every global load of the job is issued before the first wait: l[]
+        // (lane-strided), then the lane's own chunk [i0, i1) of keys and bodies
         const int chunk = (cnt + 63) >> 6;
         const int i0 = min(cnt, lane * chunk), i1 = min(cnt, i0 + chunk);
+        int lv[C];
+        uint64_t kq[C];
+        int pq[C];
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            const int i = lane + 64 * q;
+            lv[q] = i < cnt - 1 ? (int)lcp[a + i] : L0 - 1;
+        }
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            const bool in = i0 + q < i1;
+            kq[q] = in ? skeys[a + i0 + q] : 0ull;
+            pq[q] = in ? perm[a + i0 + q] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < C; ++q)
+            if (lane + 64 * q < cnt) s_min[0][lane + 64 * q] = (uint8_t)lv[q];
+        __syncthreads();
         int mine = 0, deep = L0;
-        for (int i = i0; i < i1; ++i) {
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            const int i = i0 + q;
+            if (i >= i1) break;
             const int P = i == 0 ? L0 - 1 : (int)s_min[0][i - 1];
             const int l = s_min[0][i];
             mine += max(0, l - P);
@@ -452,7 +470,10 @@ __global__ void __launch_bounds__(64) subtree_build_kernel(
         int ks;
         int run = wave_exclusive_sum(mine, &ks);
         for (int o = 32; o > 0; o >>= 1) deep = max(deep, __shfl_xor(deep, o));
-        for (int i = i0; i < i1; ++i) {
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            const int i = i0 + q;
+            if (i >= i1) break;
             const int P = i == 0 ? L0 - 1 : (int)s_min[0][i - 1];
             s_cs[i] = (uint16_t)run;
             run += max(0, (int)s_min[0][i] - P);
@@ -475,22 +496,32 @@ __global__ void __launch_bounds__(64) subtree_build_kernel(
             for (int x = lane; x + (1 << j) <= cnt - 1; x += 64)
                 s_min[j][x] = (uint8_t)min((int)s_min[j - 1][x], (int)s_min[j - 1][x + h]);
         }
-        // empty slots and cell levels
+        // empty slots, cell levels and starts
         int *cslots = child + 8 * (int64_t)base;
         for (int k = lane; k < 8 * ks; k += 64) cslots[k] = -1;
-        for (int i = i0; i < i1; ++i) {
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            const int i = i0 + q;
+            if (i >= i1) break;
             const int P = i == 0 ? L0 - 1 : (int)s_min[0][i - 1];
             const int l = s_min[0][i];
-            for (int L = P + 1; L <= l; ++L) cell_level[base + (int)s_cs[i] + (L - P - 1)] = L;
+            for (int L = P + 1; L <= l; ++L) {
+                const int c = base + (int)s_cs[i] + (L - P - 1);
+                cell_level[c] = L;
+                start[c] = a + i;
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __syncthreads();
         // every cell (but the root) and every body into its parent's slot
         bool bad = false;
-        for (int i = i0; i < i1; ++i) {
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            const int i = i0 + q;
+            if (i >= i1) break;
             const int P = i == 0 ? L0 - 1 : (int)s_min[0][i - 1];
             const int l = s_min[0][i];
-            const uint64_t key = skeys[a + i];
+            const uint64_t key = kq[q];
             const int own = (int)s_cs[i] - P - 1;  // id(i, L) - base = own + L
             // start of the level-P run through i (only used for i >= 1)
             int s = 0;
@@ -514,14 +545,16 @@ __global__ void __launch_bounds__(64) subtree_build_kernel(
             const int par = l > P ? own + l : up;
             const int d = D < kKeyLevels ? digit_at(key, D) : (l > P ? 0 : i - s);
             if (par < 0 || par >= ks || d >= 8) { bad = true; continue; }
-            child[8 * ((int64_t)base + par) + d] = perm[a + i];
+            child[8 * ((int64_t)base + par) + d] = pq[q];
         }
         if (__ballot(bad) && lane == 0) atomicOr(&st->err, kErrBucket);
         if (lane == 0) {
             child[slot] = base;
             subcells[job] = make_int4(base, ks, L0, deep);
         }
-        __syncthreads();
+        // one wave per workgroup: its LDS operations are in order, so the next
+        // job may overwrite the tables without waiting for these stores
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -532,9 +565,13 @@ struct CellSum {
 };
 
 // Sums the children of c in slot order; a level-21 bucket first puts its
-// bodies (equal keys) in ascending index order, the stable key order.
-__device__ inline CellSum sum_cell(int c, bool bucket, int n, int *child, const float *x, const float *y,
-                                   const float *z, const float *mass, const int32_t *pos, const double *mass64,
+// bodies (equal keys) in ascending index order, the stable key order.  The
+// children of a cell in slot order are consecutive ranges of the sorted
+// bodies, so with the cell's first sorted position known (first >= 0) a body
+// child is read from the sorted copy at a running cursor; otherwise (first < 0)
+// from its own index, with its sorted position from pos[].
+__device__ inline CellSum sum_cell(int c, int first, bool bucket, int n, int *child, const float4 *bodies4,
+                                   const float4 *sorted4, const int32_t *pos, const double *mass64,
                                    const double *mom64, const int *count, const int *start)
 {
     int *slots = child + 8 * (int64_t)c;
@@ -549,24 +586,29 @@ __device__ inline CellSum sum_cell(int c, bool bucket, int n, int *child, const 
         for (int s = 0; s < 8; ++s) slots[s] = v[s];
     }
     CellSum r{0.0, 0.0, 0.0, 0.0, 0, 0x7fffffff};
+    int cur = first;
     for (int s = 0; s < 8; ++s) {
         const int ch = slots[s];
         if (ch == -1) continue;
         if (ch < n) {
-            const double mb = (double)mass[ch];
+            const float4 b = cur >= 0 && cur < n ? sorted4[cur] : bodies4[ch];
+            const double mb = (double)b.w;
             r.ms = __dadd_rn(r.ms, mb);
-            r.mx = __dadd_rn(r.mx, __dmul_rn(mb, (double)x[ch]));
-            r.my = __dadd_rn(r.my, __dmul_rn(mb, (double)y[ch]));
-            r.mz = __dadd_rn(r.mz, __dmul_rn(mb, (double)z[ch]));
+            r.mx = __dadd_rn(r.mx, __dmul_rn(mb, (double)b.x));
+            r.my = __dadd_rn(r.my, __dmul_rn(mb, (double)b.y));
+            r.mz = __dadd_rn(r.mz, __dmul_rn(mb, (double)b.z));
             r.cnt += 1;
-            r.first = min(r.first, pos[ch]);
+            r.first = min(r.first, cur >= 0 ? cur : pos[ch]);
+            if (cur >= 0) cur += 1;
         } else {
             r.ms = __dadd_rn(r.ms, mass64[ch]);
             r.mx = __dadd_rn(r.mx, mom64[3 * (int64_t)ch + 0]);
             r.my = __dadd_rn(r.my, mom64[3 * (int64_t)ch + 1]);
             r.mz = __dadd_rn(r.mz, mom64[3 * (int64_t)ch + 2]);
-            r.cnt += count[ch];
+            const int k = count[ch];
+            r.cnt += k;
             r.first = min(r.first, start[ch]);
+            if (cur >= 0) cur += k;
         }
     }
     return r;
@@ -575,20 +617,22 @@ __device__ inline CellSum sum_cell(int c, bool bucket, int n, int *child, const 
 struct SumOut {
     int n;
     int *child;
-    const float *x, *y, *z;  // bodies
-    float *mass;             // bodies read, cells written (single system)
+    const float4 *bodies4;   // bodies (x, y, z, m) by index
+    float *mass;             // cells written (single system)
     float *cx, *cy, *cz;     // cell centre of mass (single system; nullptr for the ensemble)
-    int *count, *start;
+    int *count, *start;      // start: set by the build for every cell when `cursor`
     const int32_t *pos, *cell_level;
     double *mass64, *mom64;
-    float4 *nodes4;
+    float4 *nodes4;          // bodies in key order at [0, n) (read), cells at [n, m) (written)
+    int cursor;              // cell starts known before the summary
 };
 
-__device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool single_root)
+__device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool single_root, int root_first = 0)
 {
     const bool bucket = !single_root && o.cell_level[c] == kKeyLevels;
-    const CellSum r = sum_cell(c, bucket, o.n, o.child, o.x, o.y, o.z, o.mass, o.pos, o.mass64, o.mom64, o.count,
-                               o.start);
+    const int first = !o.cursor ? -1 : single_root ? root_first : o.start[c];
+    const CellSum r = sum_cell(c, first, bucket, o.n, o.child, o.bodies4, o.nodes4, o.pos, o.mass64, o.mom64,
+                               o.count, o.start);
     if (r.cnt > 0 && !(r.ms > 0.0)) atomicOr(&st->err, kErrNonFinite);
     const double gx = r.mx / r.ms, gy = r.my / r.ms, gz = r.mz / r.ms;
     const float fx = (float)gx, fy = (float)gy, fz = (float)gz, fm = (float)r.ms;
@@ -647,19 +691,23 @@ __global__ void __launch_bounds__(1024) top_summary_kernel(SumOut o, Segs g, int
 }
 
 // ------------------------------------------------------------------ layout
-// Bodies in key order as float4 (x, y, z, m) at nodes4[0..n) and sorted[];
-// then the frame blocks of the force walk (frame_layout_kernel).
-__global__ void __launch_bounds__(256) layout_bodies_kernel(int n, const int32_t *__restrict__ perm,
+// Bodies in key order as float4 (x, y, z, m) at nodes4[0..n), made right after
+// the build for the summary; sorted[] (the reference's output of the sort
+// stage) in the sort stage; then the frame blocks of the force walk
+// (frame_layout_kernel).
+__global__ void __launch_bounds__(256) sorted_bodies_kernel(int n, const int32_t *__restrict__ perm,
                                                             const float4 *__restrict__ bodies4,
-                                                            float4 *__restrict__ nodes4, int *__restrict__ sorted,
-                                                            const DeviceState *st)
+                                                            float4 *__restrict__ nodes4, const DeviceState *st)
 {
     if (latched(st)) return;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += blockDim.x * gridDim.x) {
-        const int b = perm[k];
-        sorted[k] = b;
-        nodes4[k] = bodies4[b];
-    }
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += blockDim.x * gridDim.x) nodes4[k] = bodies4[perm[k]];
+}
+
+__global__ void __launch_bounds__(256) sorted_index_kernel(int n, const int32_t *__restrict__ perm,
+                                                           int *__restrict__ sorted, const DeviceState *st)
+{
+    if (latched(st)) return;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += blockDim.x * gridDim.x) sorted[k] = perm[k];
 }
 
 // Frame blocks.  Block b lists the non-empty children of one cell, compacted in
@@ -813,7 +861,7 @@ constexpr int kPersistentBlocks = 2048;  // 256 CUs x 8 workgroups
 
 static void build_tree(Segs g, int nsamples, int base0, int m, TreeScratch &ts, DeviceState *st, hipStream_t s,
                        int n, const uint64_t *skeys, const int32_t *perm, const uint64_t *keys, int *child,
-                       int *index, int32_t *cell_level, bool locked)
+                       int *index, int32_t *cell_level, int *start, bool locked)
 {
     lcp_kernel<<<capped_grid(n), 256, 0, s>>>(n, skeys, ts.lcp, st);
     top_cells_kernel<<<(nsamples + 255) / 256, 256, 0, s>>>(g, nsamples, skeys, ts.top_flag, ts.top_id, ts.top_a,
@@ -822,13 +870,13 @@ static void build_tree(Segs g, int nsamples, int base0, int m, TreeScratch &ts, 
     const int wire = nsamples * kLevels + g.nseg;
     top_wire_kernel<<<(wire + 255) / 256, 256, 0, s>>>(g, nsamples, skeys, perm, child, cell_level, ts.top_flag,
                                                         ts.top_id, ts.top_a, ts.top_b, ts.sub, ts.sub_max, ts.top_list,
-                                                        base0, m, index, st);
+                                                        base0, m, index, start, st);
     if (locked)
         subtree_insert_kernel<<<kPersistentBlocks, 256, 0, s>>>(n, m, skeys, perm, keys, ts.lcp, child, index,
                                                                 cell_level, ts.sub, ts.subcells, st);
     else
-        subtree_build_kernel<<<kBuildBlocks, 64, 0, s>>>(m, skeys, perm, ts.lcp, child, index, cell_level, ts.sub,
-                                                         ts.sub_max, ts.subcells, st);
+        subtree_build_kernel<<<kBuildBlocks, 64, 0, s>>>(m, skeys, perm, ts.lcp, child, index, cell_level, start,
+                                                         ts.sub, ts.sub_max, ts.subcells, st);
 }
 
 static void summarise_tree(Segs g, int nsamples, TreeScratch &ts, DeviceState *st, hipStream_t s,
@@ -871,21 +919,22 @@ void launch_build(nbx_context *c, hipStream_t s, int n, int m, int *child, int *
     if (n <= 0) return;
     Segs g{nullptr, nullptr, 1, n, 0};
     build_tree(g, samples_of(n), n, m, c->ts, c->st, s, n, c->skeys, c->perm, c->keys, child, index, c->cell_level,
-               c->p.build_locked != 0);
+               c->start, c->p.build_locked != 0);
+    sorted_bodies_kernel<<<capped_grid(n), 256, 0, s>>>(n, c->perm, c->bodies4, c->nodes4, c->st);
 }
 
 void launch_summarize(nbx_context *c, hipStream_t s)
 {
     Segs g{nullptr, nullptr, 1, c->n, 0};
-    SumOut o{c->n, c->child, c->x, c->y, c->z, c->mass, c->x, c->y, c->z, c->count, c->start, c->pos,
-             c->cell_level, c->mass64, c->mom64, c->nodes4};
+    SumOut o{c->n, c->child, c->bodies4, c->mass, c->x, c->y, c->z, c->count, c->start, c->pos,
+             c->cell_level, c->mass64, c->mom64, c->nodes4, c->p.build_locked == 0};
     summarise_tree(g, samples_of(c->n), c->ts, c->st, s, o);
 }
 
 void launch_layout(nbx_context *c, hipStream_t s, int *sorted)
 {
     const int n = c->n;
-    layout_bodies_kernel<<<capped_grid(n), 256, 0, s>>>(n, c->perm, c->bodies4, c->nodes4, sorted, c->st);
+    sorted_index_kernel<<<capped_grid(n), 256, 0, s>>>(n, c->perm, sorted, c->st);
     launch_frame_layout(c, s);
     group_keys_kernel<<<capped_grid(n), 256, 0, s>>>(n, c->skeys, c->perm, nullptr, c->gkey, c->st);
     launch_radix_passes(c->hist, s, c->gkey, nullptr, n, 0, group_passes(0), c->gkey2, c->gorder, c->kbuf, c->vbuf,
@@ -905,20 +954,21 @@ void launch_batch_build(nbx_batch *b, hipStream_t s)
 {
     Segs g{b->off, b->samp_off, b->S, b->N, 1};
     build_tree(g, b->ts.samples_max, b->N + b->S, b->m, b->ts, b->st, s, b->N, b->skeys, b->perm, b->keys, b->child,
-               b->index, b->cell_level, b->p.build_locked != 0);
+               b->index, b->cell_level, b->start, b->p.build_locked != 0);
+    sorted_bodies_kernel<<<capped_grid(b->N), 256, 0, s>>>(b->N, b->perm, b->bodies4, b->nodes4, b->st);
 }
 
 void launch_batch_summarize(nbx_batch *b, hipStream_t s)
 {
     Segs g{b->off, b->samp_off, b->S, b->N, 1};
-    SumOut o{b->N, b->child, b->x, b->y, b->z, b->mass, nullptr, nullptr, nullptr, b->count, b->start, b->pos,
-             b->cell_level, b->mass64, b->mom64, b->nodes4};
+    SumOut o{b->N, b->child, b->bodies4, b->mass, nullptr, nullptr, nullptr, b->count, b->start, b->pos,
+             b->cell_level, b->mass64, b->mom64, b->nodes4, b->p.build_locked == 0};
     summarise_tree(g, b->ts.samples_max, b->ts, b->st, s, o);
 }
 
 void launch_batch_layout(nbx_batch *b, hipStream_t s)
 {
-    layout_bodies_kernel<<<capped_grid(b->N), 256, 0, s>>>(b->N, b->perm, b->bodies4, b->nodes4, b->sorted, b->st);
+    sorted_index_kernel<<<capped_grid(b->N), 256, 0, s>>>(b->N, b->perm, b->sorted, b->st);
     SysInfo si{b->start, b->perm, b->sys_of, b->theta, b->ext};
     frame_layout_kernel<true><<<capped_grid(8 * (int64_t)(b->m - b->N)), 256, 0, s>>>(
         b->N, b->S, b->index, b->child, b->pos, b->cell_level, b->nodes4, b->bodies4, b->fnode, b->fmeta, b->froot4,
