@@ -12,7 +12,9 @@
   checkpoint with one context per step, so that every step's tree is still
   held when its vector-Jacobian product is taken (no second build).  The last
   step of a re-run segment only builds its tree: the state after it is the
-  next checkpoint (or the final state), already known.
+  next checkpoint (or the final state), already known.  The forward's last
+  segment is not re-run at all: its trees are still in the contexts (each
+  context remembers which rollout step its tree belongs to).
 
 No counterpart in the reference; the force and update arithmetic is that of
 the plain method.
@@ -34,18 +36,20 @@ class Slots:
         self.params = params if params is not None else default_params()
         z = np.zeros(self.n, np.float32)
         self.bh = [BarnesHut(z, z, z, z, z, z, z, params=self.params, device=device) for _ in range(count)]
+        self.tag = [None] * count  # what slot k's tree belongs to (None: unknown)
 
-    def tree(self, k: int, x, y, z, m):
-        """Builds slot k's tree for (x, y, z, m)."""
+    def tree(self, k: int, x, y, z, m, tag=None):
+        """Builds slot k's tree for (x, y, z, m), remembered under `tag`."""
         bh, n = self.bh[k], self.n
         for key, t in (("x", x), ("y", y), ("z", z), ("mass", m)):
             bh.arr[key][:n].copy_(t)
         bh.rebuild()
+        self.tag[k] = tag
 
-    def accel(self, k: int, x, y, z, m):
+    def accel(self, k: int, x, y, z, m, tag=None):
         """Builds slot k's tree for (x, y, z, m) and returns its accelerations (new tensors)."""
         bh, n = self.bh[k], self.n
-        self.tree(k, x, y, z, m)
+        self.tree(k, x, y, z, m, tag)
         bh.forces()
         return bh.arr["ax"][:n].clone(), bh.arr["ay"][:n].clone(), bh.arr["az"][:n].clone()
 
@@ -79,9 +83,9 @@ class BarnesHutAccel(torch.autograd.Function):
         return gx, gy, gz, gm, None
 
 
-def _step(slots: Slots, k: int, s, m, dt: float):
+def _step(slots: Slots, k: int, s, m, dt: float, tag=None):
     x, y, z, vx, vy, vz = s
-    ax, ay, az = slots.accel(k, x, y, z, m)
+    ax, ay, az = slots.accel(k, x, y, z, m, tag)
     vx = torch.add(vx, ax, alpha=dt)
     vy = torch.add(vy, ay, alpha=dt)
     vz = torch.add(vz, az, alpha=dt)
@@ -96,11 +100,12 @@ class Rollout(torch.autograd.Function):
     def forward(ctx, x, y, z, vx, vy, vz, m, slots: Slots, steps: int, dt: float, every: int):
         s = (x, y, z, vx, vy, vz)
         ckpt = []
+        run = object()  # names this rollout's trees in the slots
         for k in range(steps):
             if k % every == 0:
                 ckpt.append(tuple(t.clone() for t in s))
-            s = _step(slots, k % every, s, m, dt)
-        ctx.slots, ctx.steps, ctx.dt, ctx.every = slots, steps, dt, every
+            s = _step(slots, k % every, s, m, dt, (run, k))
+        ctx.slots, ctx.steps, ctx.dt, ctx.every, ctx.run = slots, steps, dt, every, run
         ctx.ckpt = ckpt
         ctx.save_for_backward(m)
         return tuple(t.clone() for t in s)
@@ -116,12 +121,13 @@ class Rollout(torch.autograd.Function):
         for seg in reversed(range(nseg)):
             k0 = seg * every
             k1 = min(steps, k0 + every)
-            # re-run the segment: slot j keeps the tree of state k0 + j; the last
-            # state's tree is all its step needs (the state after it is known)
-            s = ctx.ckpt[seg]
-            for k in range(k0, k1 - 1):
-                s = _step(slots, k - k0, s, m, dt)
-            slots.tree(k1 - 1 - k0, s[0], s[1], s[2], m)
+            if any(slots.tag[k - k0] != (ctx.run, k) for k in range(k0, k1)):
+                # re-run the segment: slot j keeps the tree of state k0 + j; the
+                # last state's tree is all its step needs (the state after it is known)
+                s = ctx.ckpt[seg]
+                for k in range(k0, k1 - 1):
+                    s = _step(slots, k - k0, s, m, dt, (ctx.run, k))
+                slots.tree(k1 - 1 - k0, s[0], s[1], s[2], m, (ctx.run, k1 - 1))
             for k in reversed(range(k0, k1)):
                 # x_{k+1} = x_k + dt v_{k+1}, v_{k+1} = v_k + dt a(x_k)
                 lvx = torch.add(lvx, lx, alpha=dt)

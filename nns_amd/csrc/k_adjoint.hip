@@ -6,7 +6,9 @@
 // node n of mass M at X (d = X - x_i, s = (|d|^2 + eps^2)^-3/2),
 //     h      = M s (u - 3 (u.d) d / (|d|^2 + eps^2))      = d(u.a_i)/dX
 //     dL/dx_i -= h,   dL/dX_n += h,   dL/dM_n += s (u.d).
-// A zero distance contributes nothing (the forward skips it).  The node terms
+// The lane's own body contributes nothing (its forward term vanishes
+// identically); another body at the same point does when eps > 0 (h = M eps^-3
+// u) and is skipped, as in the forward walk, when eps = 0.  The node terms
 // of one child step are summed over the lanes that took it, through LDS: each
 // step writes one float4 per lane (rows padded to 65 entries, so the
 // transposed reads below are free of bank conflicts), and every kVjpBatch = 4
@@ -75,6 +77,7 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
     const int p = valid ? gorder[first + lane] : 0;  // the lane's body, by sorted position
     const float4 me = valid ? nodes4[p] : make_float4(0.f, 0.f, 0.f, 0.f);
     const int i = valid ? perm[p] : 0;
+    const int me_tag = valid ? p << 4 : -1;  // the descriptor tag of the lane's own body
     const float vx = valid ? G * ux[i] : 0.f;
     const float vy = valid ? G * uy[i] : 0.f;
     const float vz = valid ? G * uz[i] : 0.f;
@@ -136,13 +139,17 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
         const float dx = q.x - me.x;
         const float dy = q.y - me.y;
         const float dz = q.z - me.z;
-        const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        const uint64_t ok = __ballot(crit < d2);
+        const float d2e = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));  // |d|^2 + eps^2
+        const uint64_t ok = __ballot(crit < d2e);  // the force walk's test
         const uint64_t omask = mask & ~ok;
-        const uint64_t takem = mask & ok & __ballot(d2 > 0.f);
+        // The lane's own term vanishes identically (a body's tag is its sorted
+        // position << 4); another body at the same point keeps its derivative
+        // when eps > 0, and is left out, as in the forward walk, when eps = 0.
+        const uint64_t keep = eps2 > 0.f ? ~__ballot(tag == me_tag) : __ballot(d2e > 0.f);
+        const uint64_t takem = mask & ok & keep;
         if (takem) {
             const bool sel = __builtin_amdgcn_inverse_ballot_w64(takem);
-            const float r2 = sel ? d2 + eps2 : 1.f;  // finite on every lane
+            const float r2 = sel ? d2e : 1.f;  // finite on every lane
             const float rs = __builtin_amdgcn_rsqf(r2);
             const float rs2 = rs * rs;
             const float s3 = sel ? rs * rs2 : 0.f;
@@ -179,10 +186,12 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
             sp = 1;
         }
     }
-    while (sp > 0) {
+    bool overflow = false;
+    while (sp > 0) {  // one exit: the accumulators keep their registers between frames
         if (sp > kForceStack - 7) {
-            if (lane == 0) atomicOr(err, kErrStack);
-            break;
+            overflow = true;
+            sp = 0;
+            continue;
         }
         sp = __builtin_amdgcn_readfirstlane(sp - 1);
         const uint4 e = s[sp];
@@ -221,6 +230,7 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
             }
         }
     }
+    if (overflow && lane == 0) atomicOr(err, kErrStack);
     if (nrec > 0) flush(nrec);
     if (!full)
         for (int64_t k = cpos + lane; k < cend; k += 64) log_id[k] = sentinel;  // seal the last chunk
@@ -412,7 +422,7 @@ void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy,
 {
     const int n = c->n;
     if (n <= 0) return;
-    if (c->layout_theta != c->p.theta) launch_frame_layout(c, s);
+    if (frame_layout_stale(c)) launch_frame_layout(c, s);
     const int64_t ntgt = 4 * ((int64_t)n + (c->m - n + 1));  // bodies, then blocks
     clear_f64_kernel<<<(int)std::min<int64_t>((ntgt + 255) / 256, 65536), 256, 0, s>>>(c->adj_tgt, ntgt);
     vjp_reset_kernel<<<1, 1, 0, s>>>(c->st);
@@ -422,7 +432,7 @@ void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy,
     const int ngroups = (n + 63) / 64;
     vjp_kernel<W><<<(ngroups + W - 1) / W, 64 * W, 0, s>>>(
         n, ngroups, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->perm, c->gorder, c->st,
-        c->p.softening * c->p.softening, g, ux, uy, uz, gx, gy, gz, c->adj_tgt, c->adj_log_id, c->adj_log_val,
+        layout_eps2(c->p), g, ux, uy, uz, gx, gy, gz, c->adj_tgt, c->adj_log_id, c->adj_log_val,
         c->adj_cap, sentinel, c->st, &c->st->err);
     log_count_kernel<<<1, 1, 0, s>>>(c->st, c->adj_cap);
     uint64_t *tk = passes % 2 ? c->adj_log_id : c->adj_tk;  // the input keys are dead after the first pass

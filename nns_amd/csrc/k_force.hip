@@ -12,12 +12,14 @@
 // an LDS stack of {block, nch | level << 4, lane mask}.
 //
 // Each lane applies its own opening test to a cell child (l/D < theta written
-// as crit_L < |d|^2 with crit_L = (l_L/theta)^2, the reference CPU test of
-// 3D/OpenMP/OpenMP_NBody/src/Interaction.cpp:64-67; crit_L comes with the
-// child's descriptor).  The test is a ballot, so lane masks stay in SGPRs:
-// lanes that accept keep the cell's monopole (a select on mask & accept),
-// lanes that reject push the child's frame with mask & ~accept.  Every lane therefore sees exactly its own per-particle
-// interaction list; the wave pays for the union of the 64 lists.  Bodies are
+// as crit_L + eps^2 < |d|^2 + eps^2 with crit_L = (l_L/theta)^2, the reference
+// CPU test of 3D/OpenMP/OpenMP_NBody/src/Interaction.cpp:64-67; the left side
+// comes with the child's descriptor, the right side is the interaction's own
+// |d|^2 + eps^2, so the test costs one compare).  The test is a ballot, so lane masks stay in SGPRs:
+// lanes that accept evaluate the cell's monopole (under an exec mask of
+// mask & accept), lanes that reject push the child's frame with mask & ~accept.
+// Every lane therefore sees exactly its own per-particle interaction list; the
+// wave pays for the union of the 64 lists.  Bodies are
 // always direct (Interaction.cpp:60-63); a zero distance contributes nothing
 // (:28-30).
 //
@@ -78,28 +80,30 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
     int64_t cnt = 0;
 
     // One child step: every lane of `mask` meets child q with descriptor
-    // {tag, crit}; returns the lanes that open it.  The monopole is evaluated
-    // on every lane and kept where the lane takes the child (a select, not a
-    // branch); a body's crit is -1, so it is always taken and never opened.
+    // {tag, crit}; returns the lanes that open it.  The monopole runs with the
+    // exec mask set to the lanes that take the child (no select; the whole
+    // block is skipped when none does); a body's crit is -1, so it is always
+    // taken and never opened.
     auto step = [&](const float4 q, const int tag, const float crit, const uint64_t mask) -> uint64_t {
         const float dx = q.x - me.x;
         const float dy = q.y - me.y;
         const float dz = q.z - me.z;
-        const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        const uint64_t ok = __ballot(crit < d2);
+        const float d2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));  // |d|^2 + eps^2
+        const uint64_t ok = __ballot(crit < d2);                         // crit = crit_L + eps^2
         uint64_t takem = mask & ok;
         const uint64_t omask = mask & ~ok;
         if (COUNT) {
             if (__builtin_amdgcn_inverse_ballot_w64(takem)) cnt += ((tag & 15) == 0 && (tag >> 4) == p) ? 0 : 1;
         }
         if (!SOFT) takem &= __ballot(d2 > 0.f);
-        const float r2 = d2 + eps2;
-        const float ri = __builtin_amdgcn_rsqf(r2);
-        float f = q.w * ri * (ri * ri);
-        f = __builtin_amdgcn_inverse_ballot_w64(takem) ? f : 0.f;
-        px = fmaf(f, dx, px);
-        py = fmaf(f, dy, py);
-        pz = fmaf(f, dz, pz);
+        if (__builtin_amdgcn_inverse_ballot_w64(takem)) {  // exec = takem: no select, skipped when empty
+            asm volatile("");  // keeps the branch (no if-conversion into selects of the partials)
+            const float ri = __builtin_amdgcn_rsqf(d2);
+            const float f = q.w * ri * (ri * ri);
+            px = fmaf(f, dx, px);
+            py = fmaf(f, dy, py);
+            pz = fmaf(f, dz, pz);
+        }
         return omask;
     };
 
@@ -238,7 +242,7 @@ static ForceArgs single_args(nbx_context *c, float *ax, float *ay, float *az, fl
     a.ngroups = (c->n + 63) / 64;
     a.nodes4 = c->nodes4; a.fnode = c->fnode; a.fmeta = c->fmeta; a.froot4 = c->froot4; a.frootm = c->frootm;
     a.perm = c->perm; a.gorder = c->gorder; a.st = c->st;
-    a.eps2 = c->p.softening * c->p.softening;
+    a.eps2 = layout_eps2(c->p);
     a.G = g;
     a.ax = ax; a.ay = ay; a.az = az; a.counts = counts;
     a.err = &c->st->err;
@@ -248,14 +252,14 @@ static ForceArgs single_args(nbx_context *c, float *ax, float *ay, float *az, fl
 void launch_force(nbx_context *c, hipStream_t s, float *ax, float *ay, float *az, float g, int64_t *counts)
 {
     if (c->n <= 0) return;
-    if (c->layout_theta != c->p.theta) launch_frame_layout(c, s);  // thresholds of the current theta
+    if (frame_layout_stale(c)) launch_frame_layout(c, s);  // thresholds of the current theta and eps
     dispatch<false>(single_args(c, ax, ay, az, g, counts), s, c->p.kahan != 0, c->force_waves);
 }
 
 void launch_force_waves(nbx_context *c, hipStream_t s, float *ax, float *ay, float *az, float g, int waves)
 {
     if (c->n <= 0) return;
-    if (c->layout_theta != c->p.theta) launch_frame_layout(c, s);
+    if (frame_layout_stale(c)) launch_frame_layout(c, s);
     dispatch<false>(single_args(c, ax, ay, az, g, nullptr), s, c->p.kahan != 0, waves);
 }
 
@@ -266,7 +270,7 @@ void launch_batch_force(nbx_batch *b, hipStream_t s, int64_t *counts, int waves)
     a.ngroups = b->ngroups;
     a.nodes4 = b->nodes4; a.fnode = b->fnode; a.fmeta = b->fmeta; a.froot4 = b->froot4; a.frootm = b->frootm;
     a.perm = b->perm; a.gorder = b->gorder; a.st = b->st;
-    a.eps2 = b->p.softening * b->p.softening;
+    a.eps2 = layout_eps2(b->p);
     a.G = b->p.gravity;
     a.ax = counts ? b->acc_scratch : b->ax;
     a.ay = counts ? b->acc_scratch + b->N : b->ay;

@@ -714,14 +714,15 @@ __global__ void __launch_bounds__(256) sorted_index_kernel(int n, const int32_t 
 // slot order: fnode[8b + k] = (x, y, z, m) of the child and fmeta[8b + k] its
 // descriptor {tag, crit}:
 //   body:  tag = sorted position << 4,        crit = -1 (always accepted);
-//   cell:  tag = block << 4 | children (1-8), crit = crit_L = (l_L/theta)^2,
-// with crit_L computed as in the opening test (crit_0 rounded once from FP64,
-// scaled by 4^-L exactly), so the walk compares crit < |d|^2 directly.
+//   cell:  tag = block << 4 | children (1-8), crit = crit_L + eps^2,
+// crit_L = (l_L/theta)^2 computed as in the opening test (crit_0 rounded once
+// from FP64, scaled by 4^-L exactly) and eps^2 added once in FP32, so the walk
+// compares crit < |d|^2 + eps^2 -- the value its interaction needs anyway.
 // A cell child is described by the bottom of its chain, the first descendant
 // with two or more children: the cells of a one-child chain carry the same mass
 // and centre of mass (the FP64 sums of a single child are exact copies), and
-// crit_L falls with L, so a body accepts some cell of the chain exactly when it
-// accepts the bottom one, and meets the same monopole once either way.  fnode
+// crit_L + eps^2 falls with L, so a body accepts some cell of the chain exactly
+// when it accepts the bottom one, and meets the same monopole once either way.  fnode
 // keeps the data of the child itself.
 // Single system: block 0 is the root (slots child[0..7]), block i >= 1 is cell
 // n + i - 1.  Ensemble: block i is cell n + i, the roots being blocks 0..S-1.
@@ -734,20 +735,21 @@ __device__ inline int block_of(int c, int n)
     return BATCH ? c - n : (c == 0 ? 0 : c - n + 1);
 }
 
-// Opening threshold of a level-L cell: crit_L = (l_L/theta)^2, l_0 the side of the
-// root cell of the key grid.
-__device__ inline float crit_of(double span, float theta, int lev)
+// Opening threshold of a level-L cell as the walk compares it: crit_L + eps^2,
+// crit_L = (l_L/theta)^2, l_0 the side of the root cell of the key grid.
+__device__ inline float crit_of(double span, float theta, float eps2, int lev)
 {
     const double lc0 = ldexp(span / kMaskCoord, kKeyLevels);
     const double r0 = lc0 / (double)theta;
-    return ldexpf((float)(r0 * r0), -2 * lev);
+    return ldexpf((float)(r0 * r0), -2 * lev) + eps2;
 }
 
 // Descriptor of the bottom of the one-child chain starting at cell c (c = 0 is
 // the single-system root).
 template <bool BATCH>
 __device__ inline int2 chain_bottom(int c, int n, const int *__restrict__ child,
-                                    const int32_t *__restrict__ cell_level, bool root, double span, float theta)
+                                    const int32_t *__restrict__ cell_level, bool root, double span, float theta,
+                                    float eps2)
 {
     for (;;) {
         const int4 *p = reinterpret_cast<const int4 *>(child + 8 * (int64_t)c);
@@ -762,7 +764,7 @@ __device__ inline int2 chain_bottom(int c, int n, const int *__restrict__ child,
             continue;
         }
         const int lev = root ? 0 : cell_level[c];
-        return make_int2((block_of<BATCH>(c, n) << 4) | cnt, __float_as_int(crit_of(span, theta, lev)));
+        return make_int2((block_of<BATCH>(c, n) << 4) | cnt, __float_as_int(crit_of(span, theta, eps2, lev)));
     }
 }
 
@@ -778,7 +780,8 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
     int n, int nroot, const int *__restrict__ index, const int *__restrict__ child, const int32_t *__restrict__ pos,
     const int32_t *__restrict__ cell_level, const float4 *__restrict__ nodes4, const float4 *__restrict__ bodies4,
     float4 *__restrict__ fnode, int2 *__restrict__ fmeta, float4 *__restrict__ froot4, int2 *__restrict__ frootm,
-    int32_t *__restrict__ pblk, int32_t *__restrict__ bpar, float theta, SysInfo si, const DeviceState *st)
+    int32_t *__restrict__ pblk, int32_t *__restrict__ bpar, float theta, float eps2, SysInfo si,
+    const DeviceState *st)
 {
     if (latched(st)) return;
     const int ncell = *index - n;
@@ -809,7 +812,7 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
         }
         if (k < total && item < nroot && s == 0) {
             froot4[item] = BATCH ? nodes4[node] : st->root4;
-            frootm[item] = chain_bottom<BATCH>(node, n, child, cell_level, true, span, th);
+            frootm[item] = chain_bottom<BATCH>(node, n, child, cell_level, true, span, th, eps2);
             pblk[item] = -1;
         }
         if (ch == -1) continue;
@@ -821,7 +824,7 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
             bpar[ps] = (int)item;
         } else {
             fnode[slot] = nodes4[ch];
-            fmeta[slot] = chain_bottom<BATCH>(ch, n, child, cell_level, false, span, th);
+            fmeta[slot] = chain_bottom<BATCH>(ch, n, child, cell_level, false, span, th, eps2);
             pblk[block_of<BATCH>(ch, n)] = (int)item;
         }
     }
@@ -946,8 +949,9 @@ void launch_frame_layout(nbx_context *c, hipStream_t s)
     SysInfo si{};
     frame_layout_kernel<false><<<capped_grid(8 * (int64_t)(c->m - c->n + 1)), 256, 0, s>>>(
         c->n, 1, c->index, c->child, c->pos, c->cell_level, c->nodes4, c->bodies4, c->fnode, c->fmeta, c->froot4,
-        c->frootm, c->pblk, c->bpar, c->p.theta, si, c->st);
+        c->frootm, c->pblk, c->bpar, c->p.theta, layout_eps2(c->p), si, c->st);
     c->layout_theta = c->p.theta;
+    c->layout_eps2 = layout_eps2(c->p);
 }
 
 void launch_batch_build(nbx_batch *b, hipStream_t s)
@@ -972,7 +976,7 @@ void launch_batch_layout(nbx_batch *b, hipStream_t s)
     SysInfo si{b->start, b->perm, b->sys_of, b->theta, b->ext};
     frame_layout_kernel<true><<<capped_grid(8 * (int64_t)(b->m - b->N)), 256, 0, s>>>(
         b->N, b->S, b->index, b->child, b->pos, b->cell_level, b->nodes4, b->bodies4, b->fnode, b->fmeta, b->froot4,
-        b->frootm, b->pblk, b->bpar, 0.f, si, b->st);
+        b->frootm, b->pblk, b->bpar, 0.f, layout_eps2(b->p), si, b->st);
     int sys_bits = 0;
     while ((1 << sys_bits) < b->S) ++sys_bits;
     group_keys_kernel<<<capped_grid(b->N), 256, 0, s>>>(b->N, b->skeys, b->perm, b->sys_of, b->gkey, b->st);

@@ -105,6 +105,7 @@ struct nbx_context {
     int32_t *adj_sv = nullptr, *adj_tv = nullptr;
     uint32_t *adj_hist = nullptr;
     float layout_theta = -1.f;   // theta the frame blocks' thresholds were computed with
+    float layout_eps2 = -1.f;    // and eps^2
     int64_t *counts = nullptr;
     float *acc_scratch = nullptr;
     float4 *bodies4 = nullptr;  // (x, y, z, m) of every body in input order
@@ -202,8 +203,15 @@ void launch_radix_passes(uint32_t *hist, hipStream_t s, const uint64_t *keys_in,
 void launch_build(nbx_context *c, hipStream_t s, int n, int m, int *child, int *index);
 void launch_summarize(nbx_context *c, hipStream_t s);
 void launch_layout(nbx_context *c, hipStream_t s, int *sorted);
-// Frame blocks of the force walk for the current theta (part of launch_layout).
+// Frame blocks of the force walk for the current theta and softening (part of
+// launch_layout); their thresholds hold crit_L + eps^2.
 void launch_frame_layout(nbx_context *c, hipStream_t s);
+// eps^2 as the walks add it (the same FP32 product everywhere).
+inline float layout_eps2(const nbx_params &p) { return p.softening * p.softening; }
+inline bool frame_layout_stale(const nbx_context *c)
+{
+    return c->layout_theta != c->p.theta || c->layout_eps2 != layout_eps2(c->p);
+}
 // Hilbert digits kept for the force groups' order (top levels of the key).
 constexpr int kGroupLevels = 10;
 

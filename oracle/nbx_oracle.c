@@ -335,7 +335,8 @@ void orc_tree_export(const orc_tree *t, int32_t *level, int32_t *start, int32_t 
 }
 
 /* Opening threshold of the FP32 predicate used on the GPU: crit_L = (l_L/theta)^2
- * as a float, crit_0 rounded once from FP64 and scaled by exact powers of 4. */
+ * as a float, crit_0 rounded once from FP64 and scaled by exact powers of 4; the
+ * GPU compares crit_L + eps^2 (one float addition) with |d|^2 + eps^2. */
 float orc_crit0(double lcell0, double theta)
 {
     double r = lcell0 / theta;
@@ -345,8 +346,10 @@ float orc_crit0(double lcell0, double theta)
 /* Per-particle Barnes-Hut walk (Interaction::treeInteraction, Interaction.cpp:58-98)
  * over the key-prefix tree with the Plummer kernel m d (|d|^2+eps2)^-3/2 and G = 1.
  *   mac_mode 0: reference predicate in FP64, l/D < theta, cell data in FP64.
- *   mac_mode 1: the GPU's FP32 predicate crit_L < fma(dz,dz,fma(dy,dy,dx*dx)) on float
- *               data, cell data taken as the float values the GPU stores.
+ *   mac_mode 1: the GPU's FP32 predicate
+ *                   crit_L + eps2 < fma(dz,dz,fma(dy,dy,fma(dx,dx,eps2)))
+ *               (l/D < theta with eps^2 added to both sides) on float data, cell
+ *               data taken as the float values the GPU stores.
  * Accumulation is FP64 in both modes.  ninter[i] counts the accepted cells plus the
  * leaf bodies other than i itself (the per-particle interaction count). */
 int orc_tree_accel(const orc_tree *t, double theta, double eps2, int mac_mode,
@@ -360,10 +363,11 @@ int orc_tree_accel(const orc_tree *t, double theta, double eps2, int mac_mode,
         return 0;
     }
     const float crit0 = orc_crit0(t->lcell0, theta);
+    const float eps2f = (float)eps2;
     float critf[ORC_KEY_LEVELS + 2];
     double lcell[ORC_KEY_LEVELS + 2];
     for (int L = 0; L <= ORC_KEY_LEVELS + 1; ++L) {
-        critf[L] = ldexpf(crit0, -2 * L);
+        critf[L] = ldexpf(crit0, -2 * L) + eps2f;
         lcell[L] = ldexp(t->lcell0, -L);
     }
     #pragma omp parallel for schedule(dynamic, 64)
@@ -395,7 +399,7 @@ int orc_tree_accel(const orc_tree *t, double theta, double eps2, int mac_mode,
                 } else {
                     float dx = t->c32[4 * c] - pxf, dy = t->c32[4 * c + 1] - pyf,
                           dz = t->c32[4 * c + 2] - pzf;
-                    float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    float d2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2f)));
                     accept = critf[L] < d2;
                     mj = t->c32[4 * c + 3]; qx = t->c32[4 * c]; qy = t->c32[4 * c + 1];
                     qz = t->c32[4 * c + 2];
@@ -441,10 +445,11 @@ int orc_tree_vjp(const orc_tree *t, double theta, double eps2, int mac_mode, con
     if (n < 2) return 0;
     double *G = calloc((size_t)(4 * nc), sizeof(double)); /* Gx, Gy, Gz, GM per cell */
     const float crit0 = orc_crit0(t->lcell0, theta);
+    const float eps2f = (float)eps2;
     float critf[ORC_KEY_LEVELS + 2];
     double lcell[ORC_KEY_LEVELS + 2];
     for (int L = 0; L <= ORC_KEY_LEVELS + 1; ++L) {
-        critf[L] = ldexpf(crit0, -2 * L);
+        critf[L] = ldexpf(crit0, -2 * L) + eps2f;
         lcell[L] = ldexp(t->lcell0, -L);
     }
     for (int64_t i = 0; i < n; ++i) {
@@ -470,7 +475,7 @@ int orc_tree_vjp(const orc_tree *t, double theta, double eps2, int mac_mode, con
                     mj = t->mass[c]; qx = t->com[3 * c]; qy = t->com[3 * c + 1]; qz = t->com[3 * c + 2];
                 } else {
                     float dx = t->c32[4 * c] - pxf, dy = t->c32[4 * c + 1] - pyf, dz = t->c32[4 * c + 2] - pzf;
-                    accept = critf[L] < fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    accept = critf[L] < fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2f)));
                     mj = t->c32[4 * c + 3]; qx = t->c32[4 * c]; qy = t->c32[4 * c + 1]; qz = t->c32[4 * c + 2];
                 }
                 if (!accept) {
@@ -481,9 +486,12 @@ int orc_tree_vjp(const orc_tree *t, double theta, double eps2, int mac_mode, con
                     continue;
                 }
             }
+            /* The body's own term vanishes identically; another body at the same
+             * point still has a derivative (m eps^-3) when eps > 0. */
+            if (ref == i) continue;
             double dx = qx - px, dy = qy - py, dz = qz - pz;
             double d2 = dx * dx + dy * dy + dz * dz;
-            if (d2 == 0.0) continue;
+            if (d2 == 0.0 && eps2 == 0.0) continue;
             double r2 = d2 + eps2;
             double s = 1.0 / (r2 * sqrt(r2));
             double ud = ux * dx + uy * dy + uz * dz;
