@@ -364,11 +364,11 @@ __global__ void __launch_bounds__(256) adj_bodies_kernel(int n, const int32_t *_
 }
 
 // Radix passes over the node ids: ids are < m_max + 1 and the sentinel,
-// 2^(8 passes) - 1, must exceed them.
+// 2^(kRadixBits passes) - 1, must exceed them.
 static int sort_passes(int64_t m_max)
 {
     int p = 1;
-    while ((int64_t(1) << (8 * p)) < m_max + 2) ++p;
+    while ((int64_t(1) << (kRadixBits * p)) < m_max + 2) ++p;
     return p;
 }
 
@@ -427,7 +427,7 @@ void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy,
     clear_f64_kernel<<<(int)std::min<int64_t>((ntgt + 255) / 256, 65536), 256, 0, s>>>(c->adj_tgt, ntgt);
     vjp_reset_kernel<<<1, 1, 0, s>>>(c->st);
     const int passes = sort_passes(c->m_max);
-    const uint64_t sentinel = (uint64_t(1) << (8 * passes)) - 1;
+    const uint64_t sentinel = (uint64_t(1) << (kRadixBits * passes)) - 1;
     constexpr int W = 2;
     const int ngroups = (n + 63) / 64;
     vjp_kernel<W><<<(ngroups + W - 1) / W, 64 * W, 0, s>>>(

@@ -1,13 +1,14 @@
 // k_radix.hip -- stable LSD radix sort of (64-bit key, int32 index) pairs.
 //
-// Eight passes of 8 bits.  Each pass is three launches over tiles of 4096 keys
-// (4 waves x 16 items x 64 lanes; wave w owns the contiguous quarter w of the
+// Passes of kRadixBits = 8 bits.  Each pass is three launches over tiles of 2048
+// keys (4 waves x 8 items x 64 lanes; wave w owns the contiguous quarter w of the
 // tile, walked item by item so that lane order is element order):
 //   upsweep   per-tile digit histogram           reads 8 B/key
 //   scan      exclusive scan of each digit row       ~1 KiB per tile
 //   downsweep stable rank + scatter                reads 12 B/key, writes 12 B/key
-// Ranks inside a wave come from a 64-lane match of the digit built from eight
-// ballots (no atomics, no intra-wave waits), so equal digits keep their order.
+// Ranks inside a wave come from a 64-lane match of the digit built from
+// kRadixBits ballots (no atomics, no intra-wave waits), so equal digits keep
+// their order.
 #include "nbx_internal.h"
 
 namespace nbx {
@@ -145,10 +146,22 @@ __global__ void __launch_bounds__(kSortThreads) radix_downsweep(
     __shared__ uint32_t dbase[kRadix];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int k = tid; k < W * kRadix; k += kSortThreads) (&run[0][0])[k] = 0u;
-    {   // digit bases: exclusive scan of the digit totals (kSortThreads == kRadix)
+    {   // digit bases: exclusive scan of the digit totals, kRadix / kSortThreads digits per thread
+        constexpr int R = kRadix / kSortThreads;
+        static_assert(R * kSortThreads == kRadix, "digits per thread");
+        uint32_t v[R], sum = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            v[r] = rowsum[R * tid + r];
+            sum += v[r];
+        }
         uint32_t tot;
-        const uint32_t ex = block_exclusive_scan(rowsum[tid], dbase, &tot);
-        dbase[tid] = ex;
+        uint32_t ex = block_exclusive_scan(sum, dbase, &tot);  // dbase is the scan's scratch
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            dbase[R * tid + r] = ex;
+            ex += v[r];
+        }
     }
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kSortTile + (int64_t)wave * (kSortItems * 64);
@@ -246,10 +259,10 @@ void launch_batch_sort(nbx_batch *b, hipStream_t s)
     const int n = b->N;
     int g = (n + 255) / 256;
     if (g > 65536) g = 65536;
-    // 1. stable sort of (key, body) over the 64 key bits
-    launch_radix_passes(b->hist, s, b->keys, nullptr, n, 0, 64 / kRadixBits, b->skeys, b->perm1, b->kbuf,
-                        b->vbuf, nullptr);
-    // 2. stable sort by system (16 bits cover 65536 systems)
+    // 1. stable sort of (key, body) over the 63 key bits below the constant top bit
+    launch_radix_passes(b->hist, s, b->keys, nullptr, n, 0, (kMortonSortBits + kRadixBits - 1) / kRadixBits,
+                        b->skeys, b->perm1, b->kbuf, b->vbuf, nullptr);
+    // 2. stable sort by system (two passes cover 65536 systems)
     system_key_kernel<<<g, 256, 0, s>>>(n, b->perm1, b->sys_of, b->k2);
     launch_radix_passes(b->hist, s, b->k2, b->perm1, n, 0, 2, b->k2buf, b->perm, b->kbuf, b->vbuf, nullptr);
     // 3. keys in (system, key) order and the inverse permutation
@@ -266,10 +279,10 @@ void launch_exclusive_scan_u32(uint32_t *data, int total, uint32_t *sums, hipStr
 }
 
 void launch_radix_sort(nbx_context *c, hipStream_t s, const uint64_t *keys_in, int n, uint64_t *keys_out,
-                       int32_t *perm_out, uint64_t *tmp_k, int32_t *tmp_v, int32_t *inv_out)
+                       int32_t *perm_out, uint64_t *tmp_k, int32_t *tmp_v, int32_t *inv_out, int key_bits)
 {
-    launch_radix_passes(c->hist, s, keys_in, nullptr, n, 0, 64 / kRadixBits, keys_out, perm_out, tmp_k, tmp_v,
-                        inv_out);
+    launch_radix_passes(c->hist, s, keys_in, nullptr, n, 0, (key_bits + kRadixBits - 1) / kRadixBits, keys_out,
+                        perm_out, tmp_k, tmp_v, inv_out);
 }
 
 }  // namespace nbx

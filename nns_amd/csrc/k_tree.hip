@@ -99,44 +99,43 @@ __global__ void __launch_bounds__(256) lcp_kernel(int n, const uint64_t *__restr
     }
 }
 
-// Top cells: thread per sample q (every kSubtreeMax-th sorted position of a
-// segment).  The level-L cell through q is [a, b); it is a top cell when it
-// holds more than kSubtreeMax bodies, and q owns it when no earlier sample lies
-// in it.
+// Top cells: one lane per (sample q, level L); q is every kSubtreeMax-th sorted
+// position of a segment.  The level-L cell through q is [a, b), found by two
+// binary searches over the segment; it is a top cell when it holds more than
+// kSubtreeMax bodies, and q owns it when no earlier sample lies in it.  The
+// levels of a sample are searched side by side, not one after another.
 __global__ void __launch_bounds__(256) top_cells_kernel(Segs g, int nsamples, const uint64_t *__restrict__ skeys,
                                                         uint32_t *__restrict__ flag, uint32_t *__restrict__ id,
                                                         int32_t *__restrict__ ta, int32_t *__restrict__ tb,
                                                         const DeviceState *st)
 {
     if (latched(st)) return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nsamples) return;
-    for (int L = 0; L < kLevels; ++L) {
-        flag[k * kLevels + L] = 0u;
-        id[k * kLevels + L] = 0u;
-    }
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = gt >> 5, L = gt & 31;
+    if (k >= nsamples || L >= kLevels) return;
+    flag[k * kLevels + L] = 0u;
+    id[k * kLevels + L] = 0u;
+    if (L == 0) return;  // the root is wired separately
     const int s = seg_of_sample(g, k);
     const int lo = seg_lo(g, s), hi = seg_hi(g, s);
     const int q = lo + (k - samp_lo(g, s)) * kSubtreeMax;
-    const uint64_t key = skeys[q];
-    int a = lo, b = hi;
-    for (int L = 1; L < kLevels; ++L) {
-        const uint64_t P = prefix_of(key, L);
-        a = lower_prefix(skeys, a, q + 1, P, L);
-        b = upper_prefix(skeys, q, b, P, L);
-        if (b - a <= kSubtreeMax) break;
-        if (q - kSubtreeMax < a || q == lo) {
-            flag[k * kLevels + L] = 1u;
-            id[k * kLevels + L] = 1u;
-            ta[k * kLevels + L] = a;
-            tb[k * kLevels + L] = b;
-        }
+    const uint64_t P = prefix_of(skeys[q], L);
+    const int a = lower_prefix(skeys, lo, q + 1, P, L);
+    const int b = upper_prefix(skeys, q, hi, P, L);
+    if (b - a > kSubtreeMax && (q - kSubtreeMax < a || q == lo)) {
+        flag[k * kLevels + L] = 1u;
+        id[k * kLevels + L] = 1u;
+        ta[k * kLevels + L] = a;
+        tb[k * kLevels + L] = b;
     }
 }
 
 // Wires every top cell (and every root): child ranges by digit L; a range of
 // one body is a leaf, of more than kSubtreeMax a top cell, otherwise a subtree
-// queued for the workgroup build.  Thread 0 also sets *index past the top cells.
+// queued for the subtree build.  Eight lanes per entry, lane d for digit d: each
+// finds the end of its digit's range by one binary search over the cell's
+// range and takes its start from the lane before (the searches run side by
+// side, not one after another).  Thread 0 also sets *index past the top cells.
 __global__ void __launch_bounds__(256) top_wire_kernel(
     Segs g, int nsamples, const uint64_t *__restrict__ skeys, const int32_t *__restrict__ perm, int *child,
     int32_t *__restrict__ cell_level, const uint32_t *__restrict__ flag, const uint32_t *__restrict__ id,
@@ -145,8 +144,10 @@ __global__ void __launch_bounds__(256) top_wire_kernel(
 {
     if (latched(st)) return;
     const int total = nsamples * kLevels;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) {
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = gt >> 3;  // entry: one (sample, level) of the top table, or one root
+    const int d = gt & 7;   // digit, slot
+    if (gt == 0) {
         const int ntop = (int)(id[total - 1] + flag[total - 1]);
         st->ntop = ntop;
         *index = base0 + ntop;
@@ -162,12 +163,14 @@ __global__ void __launch_bounds__(256) top_wire_kernel(
         b = tb[t];
         s = seg_of_sample(g, k);
         if (c >= m) return;  // capacity error latched by thread 0
-        cell_level[c] = L;
-        start[c] = a;
-        top_list[L * nsamples + atomicAdd(&st->topl[L], 1)] = c;
+        if (d == 0) {
+            cell_level[c] = L;
+            start[c] = a;
+            top_list[L * nsamples + atomicAdd(&st->topl[L], 1)] = c;
+        }
         if (L == kKeyLevels) {  // more than kSubtreeMax bodies on one key
-            atomicOr(&st->err, kErrBucket);
-            for (int d = 0; d < 8; ++d) child[8 * (int64_t)c + d] = -1;
+            if (d == 0) atomicOr(&st->err, kErrBucket);
+            child[8 * (int64_t)c + d] = -1;
             return;
         }
     } else if (t < total + g.nseg) {
@@ -176,30 +179,29 @@ __global__ void __launch_bounds__(256) top_wire_kernel(
         L = 0;
         a = seg_lo(g, s);
         b = seg_hi(g, s);
-        if (g.batch) start[c] = a;  // the single system's root is node 0, not a cell
+        if (g.batch && d == 0) start[c] = a;  // the single system's root is node 0, not a cell
     } else {
         return;
     }
+    // the 8 lanes of an entry are consecutive lanes of one wave and all get here
     const int lo = seg_lo(g, s);
-    int da = a;
-    for (int d = 0; d < 8; ++d) {
-        const int db = lower_digit(skeys, da, b, d + 1, L);
-        const int cnt = db - da;
-        const int64_t slot = 8 * (int64_t)c + d;
-        if (cnt == 0) {
-            child[slot] = -1;
-        } else if (cnt == 1) {
-            child[slot] = perm[da];
-        } else if (cnt > kSubtreeMax) {
-            const int kk = samp_lo(g, s) + (da - lo + kSubtreeMax - 1) / kSubtreeMax;
-            child[slot] = base0 + (int)id[kk * kLevels + L + 1];
-        } else {
-            child[slot] = -1;  // set by the subtree build
-            const int i = atomicAdd(&st->nsub, 1);
-            if (i < sub_max) sub[i] = make_int4(da, db, L + 1, (int)slot);
-            else atomicOr(&st->err, kErrCapacity);
-        }
-        da = db;
+    const int db = lower_digit(skeys, a, b, d + 1, L);
+    const int up = __shfl_up(db, 1, 8);
+    const int da = d == 0 ? a : up;
+    const int cnt = db - da;
+    const int64_t slot = 8 * (int64_t)c + d;
+    if (cnt == 0) {
+        child[slot] = -1;
+    } else if (cnt == 1) {
+        child[slot] = perm[da];
+    } else if (cnt > kSubtreeMax) {
+        const int kk = samp_lo(g, s) + (da - lo + kSubtreeMax - 1) / kSubtreeMax;
+        child[slot] = base0 + (int)id[kk * kLevels + L + 1];
+    } else {
+        child[slot] = -1;  // set by the subtree build
+        const int i = atomicAdd(&st->nsub, 1);
+        if (i < sub_max) sub[i] = make_int4(da, db, L + 1, (int)slot);
+        else atomicOr(&st->err, kErrCapacity);
     }
 }
 
@@ -867,11 +869,11 @@ static void build_tree(Segs g, int nsamples, int base0, int m, TreeScratch &ts, 
                        int *index, int32_t *cell_level, int *start, bool locked)
 {
     lcp_kernel<<<capped_grid(n), 256, 0, s>>>(n, skeys, ts.lcp, st);
-    top_cells_kernel<<<(nsamples + 255) / 256, 256, 0, s>>>(g, nsamples, skeys, ts.top_flag, ts.top_id, ts.top_a,
+    top_cells_kernel<<<(32 * nsamples + 255) / 256, 256, 0, s>>>(g, nsamples, skeys, ts.top_flag, ts.top_id, ts.top_a,
                                                              ts.top_b, st);
     launch_exclusive_scan_u32(ts.top_id, nsamples * kLevels, ts.scan_tmp, s);
-    const int wire = nsamples * kLevels + g.nseg;
-    top_wire_kernel<<<(wire + 255) / 256, 256, 0, s>>>(g, nsamples, skeys, perm, child, cell_level, ts.top_flag,
+    const int wire = nsamples * kLevels + g.nseg;  // entries, 8 lanes each
+    top_wire_kernel<<<(8 * wire + 255) / 256, 256, 0, s>>>(g, nsamples, skeys, perm, child, cell_level, ts.top_flag,
                                                         ts.top_id, ts.top_a, ts.top_b, ts.sub, ts.sub_max, ts.top_list,
                                                         base0, m, index, start, st);
     if (locked)

@@ -253,7 +253,7 @@ int nbx_build_tree(nbx_context *c, float *x, float *y, float *z, float *mass, in
     c->n = n; c->m = m;
     StageTimer t(c, s, elapsed_ms);
     nbx::launch_keys(c, s, x, y, z, mass, n, NBX_KEY_MORTON, c->keys);
-    nbx::launch_radix_sort(c, s, c->keys, n, c->skeys, c->perm, c->kbuf, c->vbuf, c->pos);
+    nbx::launch_radix_sort(c, s, c->keys, n, c->skeys, c->perm, c->kbuf, c->vbuf, c->pos, nbx::kMortonSortBits);
     nbx::launch_build(c, s, n, m, child, index);
     c->stage = 2;
     return t.finish();
@@ -347,7 +347,7 @@ int nbx_sort_keys(nbx_context *c, const uint64_t *keys, int n, uint64_t *sorted_
     hipSetDevice(c->device);
     hipStream_t s = (hipStream_t)stream;
     StageTimer t(c, s, elapsed_ms);
-    nbx::launch_radix_sort(c, s, keys, n, sorted_keys, perm, c->kbuf, c->vbuf, nullptr);
+    nbx::launch_radix_sort(c, s, keys, n, sorted_keys, perm, c->kbuf, c->vbuf, nullptr, 64);
     return t.finish();
 }
 

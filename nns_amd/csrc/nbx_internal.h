@@ -13,7 +13,9 @@ namespace nbx {
 constexpr int kKeyLevels = 21;
 constexpr double kMaskCoord = 2097151.0;
 
-// Radix sort: 8-bit digits, 256 threads x 8 items per tile.
+// Radix sort: 8-bit digits, 256 threads x 8 items per tile.  (9-bit digits would
+// save a pass on a 63-bit Morton key, but their 512-way scatter is slower: 10
+// passes of 9 bits took longer than 12 of 8 at N = 4M.)
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kSortThreads = 256;
@@ -186,13 +188,15 @@ void launch_update(hipStream_t s, int block, float *x, float *y, float *z, float
 void launch_fill_f32(hipStream_t s, float *p, float v, int64_t n);
 
 // launchers (k_radix.hip)
-// Sorts keys_in (not modified) into keys_out / perm_out; writes the inverse permutation to
-// inv_out when not null.  tmp_k / tmp_v: scratch of n elements.
+// Sorts keys_in (not modified) by their low key_bits bits into keys_out / perm_out (stable);
+// writes the inverse permutation to inv_out when not null.  tmp_k / tmp_v: scratch of n
+// elements.  Morton keys need 63 bits (bit 63 is set in every key).
 void launch_radix_sort(nbx_context *c, hipStream_t s, const uint64_t *keys_in, int n, uint64_t *keys_out,
-                       int32_t *perm_out, uint64_t *tmp_k, int32_t *tmp_v, int32_t *inv_out);
+                       int32_t *perm_out, uint64_t *tmp_k, int32_t *tmp_v, int32_t *inv_out, int key_bits);
+constexpr int kMortonSortBits = 63;
 // exclusive scan of total <= 2^20 values; sums: scratch of 1024 values
 void launch_exclusive_scan_u32(uint32_t *data, int total, uint32_t *sums, hipStream_t s);
-// npasses passes of 8 bits from bit first_shift; vals_in == nullptr means 0, 1, 2, ...
+// npasses passes of kRadixBits bits from bit first_shift; vals_in == nullptr means 0, 1, 2, ...
 // n_dev (nullable): device-side count; only the first min(n, *n_dev) keys are sorted
 // (grids are sized for n).  tmp_k may alias keys_in when npasses is odd.
 void launch_radix_passes(uint32_t *hist, hipStream_t s, const uint64_t *keys_in, const int32_t *vals_in, int n,
