@@ -572,25 +572,27 @@ struct CellSum {
 // bodies, so with the cell's first sorted position known (first >= 0) a body
 // child is read from the sorted copy at a running cursor; otherwise (first < 0)
 // from its own index, with its sorted position from pos[].
-__device__ inline CellSum sum_cell(int c, int first, bool bucket, int n, int *child, const float4 *bodies4,
-                                   const float4 *sorted4, const int32_t *pos, const double *mass64,
-                                   const double *mom64, const int *count, const int *start)
+__device__ inline CellSum sum_cell(int c, int first, bool bucket, int n, int *child, const int *lslots,
+                                   const float4 *bodies4, const float4 *sorted4, const int32_t *pos,
+                                   const double *mass64, const double *mom64, const int *count, const int *start)
 {
-    int *slots = child + 8 * (int64_t)c;
+    int v[8];  // the slots, from the LDS copy when there is one
+    const int *src = lslots ? lslots : child + 8 * (int64_t)c;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) v[s] = src[s];
     if (bucket) {
-        int v[8];
-        for (int s = 0; s < 8; ++s) v[s] = slots[s];
         for (int i = 1; i < 8; ++i) {
             int a = v[i], j = i - 1;
             while (j >= 0 && (v[j] == -1 || (a != -1 && v[j] > a))) { v[j + 1] = v[j]; --j; }
             v[j + 1] = a;
         }
-        for (int s = 0; s < 8; ++s) slots[s] = v[s];
+        for (int s = 0; s < 8; ++s) child[8 * (int64_t)c + s] = v[s];
     }
     CellSum r{0.0, 0.0, 0.0, 0.0, 0, 0x7fffffff};
     int cur = first;
+#pragma unroll
     for (int s = 0; s < 8; ++s) {
-        const int ch = slots[s];
+        const int ch = v[s];
         if (ch == -1) continue;
         if (ch < n) {
             const float4 b = cur >= 0 && cur < n ? sorted4[cur] : bodies4[ch];
@@ -629,12 +631,15 @@ struct SumOut {
     int cursor;              // cell starts known before the summary
 };
 
-__device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool single_root, int root_first = 0)
+// lev: the cell's level when the caller has it (else -1); lslots: an LDS copy of
+// its slots (else nullptr).
+__device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool single_root, int root_first = 0,
+                                 int lev = -1, const int *lslots = nullptr)
 {
-    const bool bucket = !single_root && o.cell_level[c] == kKeyLevels;
+    const bool bucket = !single_root && (lev >= 0 ? lev : o.cell_level[c]) == kKeyLevels;
     const int first = !o.cursor ? -1 : single_root ? root_first : o.start[c];
-    const CellSum r = sum_cell(c, first, bucket, o.n, o.child, o.bodies4, o.nodes4, o.pos, o.mass64, o.mom64,
-                               o.count, o.start);
+    const CellSum r = sum_cell(c, first, bucket, o.n, o.child, lslots, o.bodies4, o.nodes4, o.pos, o.mass64,
+                               o.mom64, o.count, o.start);
     if (r.cnt > 0 && !(r.ms > 0.0)) atomicOr(&st->err, kErrNonFinite);
     const double gx = r.mx / r.ms, gy = r.my / r.ms, gz = r.mz / r.ms;
     const float fx = (float)gx, fy = (float)gy, fz = (float)gz, fm = (float)r.ms;
@@ -655,11 +660,16 @@ __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool s
 
 // Subtree summary: persistent workgroups take subtrees from the queue; the
 // cells of a subtree are the contiguous ids [base, base + ks), summed deepest
-// level first with a barrier between levels.
+// level first with a barrier between levels.  A subtree of up to kSumLdsCells
+// cells first copies its cells' levels and slots into LDS (one coalesced read),
+// so a level costs the reads of the children's sums and the stores only.
+constexpr int kSumLdsCells = 512;
 __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const int4 *__restrict__ subcells,
                                                               DeviceState *st)
 {
     __shared__ int s_job;
+    __shared__ int s_lev[kSumLdsCells];
+    __shared__ int s_slot[8 * kSumLdsCells];
     if (latched(st)) return;
     const int tid = threadIdx.x;
     const int nsub = __hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -671,9 +681,17 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
         if (job >= nsub) break;
         const int4 sc = subcells[job];
         const int base = sc.x, ks = sc.y, L0 = sc.z;
+        const bool lds = ks <= kSumLdsCells;
+        if (lds) {
+            for (int k = tid; k < ks; k += blockDim.x) s_lev[k] = o.cell_level[base + k];
+            for (int k = tid; k < 8 * ks; k += blockDim.x) s_slot[k] = o.child[8 * (int64_t)base + k];
+            __syncthreads();
+        }
         for (int L = sc.w; L >= L0; --L) {
-            for (int c = base + tid; c < base + ks; c += blockDim.x)
-                if (o.cell_level[c] == L) summarise(o, c, st, false);
+            for (int k = tid; k < ks; k += blockDim.x) {
+                const int lev = lds ? s_lev[k] : o.cell_level[base + k];
+                if (lev == L) summarise(o, base + k, st, false, 0, lev, lds ? s_slot + 8 * k : nullptr);
+            }
             __syncthreads();
         }
     }
