@@ -701,9 +701,13 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
 __global__ void __launch_bounds__(1024) top_summary_kernel(SumOut o, Segs g, int nsamples,
                                                            const int32_t *__restrict__ top_list, DeviceState *st)
 {
+    __shared__ int s_cnt[kLevels];
     if (latched(st)) return;
+    if (threadIdx.x < kLevels) s_cnt[threadIdx.x] = st->topl[threadIdx.x];  // every level's count, read once
+    __syncthreads();
     for (int L = kKeyLevels; L >= 1; --L) {
-        const int cnt = st->topl[L];
+        const int cnt = s_cnt[L];
+        if (cnt == 0) continue;  // uniform: no barrier for an empty level
         for (int i = threadIdx.x; i < cnt; i += blockDim.x) summarise(o, top_list[L * nsamples + i], st, false);
         __syncthreads();
     }
@@ -798,10 +802,10 @@ struct SysInfo {
 template <bool BATCH>
 __global__ void __launch_bounds__(256) frame_layout_kernel(
     int n, int nroot, const int *__restrict__ index, const int *__restrict__ child, const int32_t *__restrict__ pos,
-    const int32_t *__restrict__ cell_level, const float4 *__restrict__ nodes4, const float4 *__restrict__ bodies4,
-    float4 *__restrict__ fnode, int2 *__restrict__ fmeta, float4 *__restrict__ froot4, int2 *__restrict__ frootm,
-    int32_t *__restrict__ pblk, int32_t *__restrict__ bpar, float theta, float eps2, SysInfo si,
-    const DeviceState *st)
+    const int32_t *__restrict__ cell_level, const int *__restrict__ count, const int *__restrict__ start,
+    const float4 *__restrict__ nodes4, float4 *__restrict__ fnode, int2 *__restrict__ fmeta,
+    float4 *__restrict__ froot4, int2 *__restrict__ frootm, int32_t *__restrict__ pblk, int32_t *__restrict__ bpar,
+    float theta, float eps2, SysInfo si, const DeviceState *st)
 {
     if (latched(st)) return;
     const int ncell = *index - n;
@@ -819,6 +823,17 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
         const uint64_t occ = __ballot(ch != -1);
         const unsigned seg = (unsigned)(occ >> (lane & ~7)) & 0xffu;
         const int rank = __popc(seg & ((1u << s) - 1u));
+        // Sorted position of a body child: the node's first sorted body plus the
+        // bodies of the slots before it (children in slot order are consecutive
+        // ranges of the key order).  8-lane exclusive sum of the children's sizes.
+        const int w = ch == -1 ? 0 : ch < n ? 1 : count[ch];
+        int off = w;
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) {
+            const int y = __shfl_up(off, o, 8);
+            if (s >= o) off += y;
+        }
+        off -= w;
         if (ch == -1 && !(k < total && item < nroot && s == 0)) continue;
         double span;
         float th;
@@ -838,8 +853,10 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
         if (ch == -1) continue;
         const int64_t slot = 8 * item + rank;
         if (ch < n) {
-            const int ps = pos[ch];
-            fnode[slot] = bodies4[ch];
+            const int first = (!BATCH && item == 0) ? 0 : start[node];  // the single root is node 0
+            const int q = first + off;
+            const int ps = q >= 0 && q < n ? q : pos[ch];
+            fnode[slot] = nodes4[ps];
             fmeta[slot] = make_int2(ps << 4, body_crit);
             bpar[ps] = (int)item;
         } else {
@@ -968,7 +985,7 @@ void launch_frame_layout(nbx_context *c, hipStream_t s)
 {
     SysInfo si{};
     frame_layout_kernel<false><<<capped_grid(8 * (int64_t)(c->m - c->n + 1)), 256, 0, s>>>(
-        c->n, 1, c->index, c->child, c->pos, c->cell_level, c->nodes4, c->bodies4, c->fnode, c->fmeta, c->froot4,
+        c->n, 1, c->index, c->child, c->pos, c->cell_level, c->count, c->start, c->nodes4, c->fnode, c->fmeta, c->froot4,
         c->frootm, c->pblk, c->bpar, c->p.theta, layout_eps2(c->p), si, c->st);
     c->layout_theta = c->p.theta;
     c->layout_eps2 = layout_eps2(c->p);
@@ -995,7 +1012,7 @@ void launch_batch_layout(nbx_batch *b, hipStream_t s)
     sorted_index_kernel<<<capped_grid(b->N), 256, 0, s>>>(b->N, b->perm, b->sorted, b->st);
     SysInfo si{b->start, b->perm, b->sys_of, b->theta, b->ext};
     frame_layout_kernel<true><<<capped_grid(8 * (int64_t)(b->m - b->N)), 256, 0, s>>>(
-        b->N, b->S, b->index, b->child, b->pos, b->cell_level, b->nodes4, b->bodies4, b->fnode, b->fmeta, b->froot4,
+        b->N, b->S, b->index, b->child, b->pos, b->cell_level, b->count, b->start, b->nodes4, b->fnode, b->fmeta, b->froot4,
         b->frootm, b->pblk, b->bpar, 0.f, layout_eps2(b->p), si, b->st);
     int sys_bits = 0;
     while ((1 << sys_bits) < b->S) ++sys_bits;
