@@ -5,7 +5,8 @@
 // tile, walked item by item so that lane order is element order):
 //   upsweep   per-tile digit histogram           reads 8 B/key
 //   scan      exclusive scan of each digit row       ~1 KiB per tile
-//   downsweep stable rank + scatter                reads 12 B/key, writes 12 B/key
+//   downsweep stable rank, the tile reordered by digit in LDS, then written out
+//             digit run by digit run (coalesced)  reads 12 B/key, writes 12 B/key
 // Ranks inside a wave come from a 64-lane match of the digit built from
 // kRadixBits ballots (no atomics, no intra-wave waits), so equal digits keep
 // their order.
@@ -144,6 +145,10 @@ __global__ void __launch_bounds__(kSortThreads) radix_downsweep(
     constexpr int W = kSortThreads / 64;
     __shared__ uint32_t run[W][kRadix];
     __shared__ uint32_t dbase[kRadix];
+    __shared__ uint32_t gdelta[kRadix];
+    __shared__ uint32_t scan_sh[kSortThreads];
+    __shared__ uint64_t s_key[kSortTile];
+    __shared__ int32_t s_val[kSortTile];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int k = tid; k < W * kRadix; k += kSortThreads) (&run[0][0])[k] = 0u;
     {   // digit bases: exclusive scan of the digit totals, kRadix / kSortThreads digits per thread
@@ -185,25 +190,62 @@ __global__ void __launch_bounds__(kSortThreads) radix_downsweep(
         if (valid && lane == __ffsll((unsigned long long)peers) - 1) run[wave][d] = before + (uint32_t)__popcll(peers);
     }
     __syncthreads();
-    // exclusive prefix over the waves of this tile, plus the tile's global offset
-    for (int d = tid; d < kRadix; d += kSortThreads) {
-        uint32_t s = hist[(int64_t)d * ntiles + blockIdx.x] + dbase[d];
-        for (int w = 0; w < W; ++w) {
-            uint32_t c = run[w][d];
-            run[w][d] = s;
-            s += c;
+    // The tile in digit order: digit d starts at lstart[d] = the tile's elements of
+    // smaller digits, wave w's elements of digit d follow those of waves < w.  Each
+    // digit's elements go to the global positions from gstart[d] (the tiles and
+    // digits before), so global = local + (gstart[d] - lstart[d]).
+    {
+        constexpr int R = kRadix / kSortThreads;
+        uint32_t cnt[R][W], tc[R], tsum = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int d = R * tid + r;
+            tc[r] = 0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                cnt[r][w] = run[w][d];
+                tc[r] += cnt[r][w];
+            }
+            tsum += tc[r];
+        }
+        uint32_t tot;
+        uint32_t ls = block_exclusive_scan(tsum, scan_sh, &tot);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int d = R * tid + r;
+            gdelta[d] = hist[(int64_t)d * ntiles + blockIdx.x] + dbase[d] - ls;
+            uint32_t o = ls;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                run[w][d] = o;
+                o += cnt[r][w];
+            }
+            ls += tc[r];
         }
     }
     __syncthreads();
+    // scatter into the tile's digit order in LDS ...
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
         const int64_t e = base + i * 64 + lane;
         if (e < n) {
-            const uint32_t p = run[wave][dig[i]] + rank[i];
-            kout[p] = key[i];
-            vout[p] = val[i];
-            if (inv) inv[val[i]] = (int32_t)p;
+            const uint32_t lp = run[wave][dig[i]] + rank[i];
+            s_key[lp] = key[i];
+            s_val[lp] = val[i];
         }
+    }
+    __syncthreads();
+    // ... and write it out in that order: consecutive threads, consecutive addresses
+    const int64_t t0 = (int64_t)blockIdx.x * kSortTile;
+    const int tn = (int)min((int64_t)kSortTile, (int64_t)n - t0);
+    for (int j = tid; j < tn; j += kSortThreads) {
+        const uint64_t k = s_key[j];
+        const int32_t v = s_val[j];
+        const uint32_t d = (uint32_t)((k >> shift) & (kRadix - 1));
+        const uint32_t p = (uint32_t)j + gdelta[d];
+        kout[p] = k;
+        vout[p] = v;
+        if (inv) inv[v] = (int32_t)p;
     }
 }
 
