@@ -269,6 +269,8 @@ def run_adjoint(args):
             "forward_ms": round((t1 - t0) * 1e3, 2), "backward_ms": round((t2 - t1) * 1e3, 2),
             "adjoint_over_forward": round((t2 - t1) / (t1 - t0), 3),
             "tree_force_ms": round(_median(fw), 4), "vjp_ms": round(_median(vj), 4),
+            # the VJP of one step (its tree held) over one forward step (tree + forces)
+            "vjp_over_forward_step": round(_median(vj) / _median(fw), 3),
             "grad_norms": {k: float(t.grad.norm().item()) for k, t in zip(("x", "y", "z", "vx", "vy", "vz", "m"),
                                                                            ins)},
             "fd_check": {"coords": len(picks), "h": h, "median_rel_err": float(np.median(rel)),
