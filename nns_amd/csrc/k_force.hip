@@ -26,7 +26,7 @@
 // Interaction: a += m d (|d|^2 + eps^2)^-3/2, d = q - x (the 20-flop form of
 // GPU Gems 3 ch. 31), with the raw v_rsq_f32 (|d|^2 + eps^2 is never denormal
 // when eps > 0; the unsoftened variant masks d = 0).  With Kahan on, monopoles
-// go into a plain partial that is folded into a compensated total every 4
+// go into a plain partial that is folded into a compensated total every 8
 // frames.  Bookkeeping that is not per child (stack bound, Kahan) is per frame:
 // the scalar unit, shared by the CU's four SIMDs, is the other limit here.
 //
@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
     // Frames run in tiles of kTile; the partials are folded after each tile.
     // The frame loop has one exit, so the accumulators keep their registers
     // from frame to frame (no copies between iterations).
-    constexpr int kTile = KAHAN ? 4 : 0x7fffffff;
+    constexpr int kTile = KAHAN ? 8 : 0x7fffffff;
     bool overflow = false;
     do {
       for (int t = 0; t < kTile; ++t) {
@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
             }
         }
       }
-        if (KAHAN) {  // fold the partials of the tile (about 20 children)
+        if (KAHAN) {  // fold the partials of the tile (about 40 children)
             kahan_add(sx, cx, px); kahan_add(sy, cy, py); kahan_add(sz, cz, pz);
             px = py = pz = 0.f;
         }
