@@ -18,13 +18,14 @@
 //
 // The per-step sums do not go to the nodes by atomics (one FP64 atomic per
 // component and step is a memory-side transaction: at N = 262K they cost more
-// than the walk).  They are appended to a log {node, float4}: each wave takes
-// chunks of kChunk records with one atomicAdd and seals the unused tail of a
-// chunk with a sentinel node id.  The log is radix-sorted by node id (stable,
-// 3 passes for up to 2^24 - 1 nodes) and summed per node in FP64, in log order,
-// by threads that each walk 64 sorted records; only the node runs that cross a
-// thread's 64 records take an atomic.  A wave that finds the log full adds its
-// remaining sums with atomics directly.
+// than the walk).  They are appended to a log {32-bit node id, float4}: each
+// wave takes chunks of kChunk records with one atomicAdd and seals the unused
+// tail of a chunk with a sentinel node id.  The log is radix-sorted by node id
+// (stable, 3 passes for up to 2^24 - 1 nodes) and summed per node in FP64, in
+// log order, over tiles of 64 sorted records (one per lane); only the node runs
+// that cross a tile take an atomic.  The log order is that of the chunk grants,
+// so a node's FP64 total can differ from run to run in its last bits.  A wave
+// that finds the log full adds its remaining sums with atomics directly.
 //
 // A cell's X and M are sums over its bodies, X = sum m_k x_k / M, so the node
 // gradients flow down: for every body k
@@ -50,7 +51,7 @@ __device__ inline float seg16_sum(float v)
     return v;
 }
 
-template <int W>
+template <int W, bool SOFT>
 __global__ void __launch_bounds__(64 * W) vjp_kernel(
     int n, int ngroups, const float4 *__restrict__ nodes4, const float4 *__restrict__ fnode,
     const int2 *__restrict__ fmeta, const float4 *__restrict__ froot4, const int2 *__restrict__ frootm,
@@ -58,7 +59,7 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
     float G,
     const float *__restrict__ ux, const float *__restrict__ uy, const float *__restrict__ uz,
     float *__restrict__ gx, float *__restrict__ gy, float *__restrict__ gz, double *__restrict__ tgt,
-    uint64_t *__restrict__ log_id, float4 *__restrict__ log_val, int64_t cap, uint64_t sentinel, DeviceState *dst,
+    uint32_t *__restrict__ log_id, float4 *__restrict__ log_val, int64_t cap, uint32_t sentinel, DeviceState *dst,
     int *__restrict__ err)
 {
     __shared__ uint4 stk[W][kForceStack];
@@ -120,7 +121,7 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
         }
         if (q_ == 0 && s_ < count) {
             if (!full) {
-                log_id[cpos + s_] = (uint64_t)ri[s_];
+                log_id[cpos + s_] = (uint32_t)ri[s_];
                 log_val[cpos + s_] = make_float4(ax, ay, az, aw);
             } else {
                 double *t = tgt + 4 * (int64_t)ri[s_];
@@ -145,11 +146,11 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
         // The lane's own term vanishes identically (a body's tag is its sorted
         // position << 4); another body at the same point keeps its derivative
         // when eps > 0, and is left out, as in the forward walk, when eps = 0.
-        const uint64_t keep = eps2 > 0.f ? ~__ballot(tag == me_tag) : __ballot(d2e > 0.f);
+        const uint64_t keep = SOFT ? ~__ballot(tag == me_tag) : __ballot(d2e > 0.f);
         const uint64_t takem = mask & ok & keep;
         if (takem) {
             const bool sel = __builtin_amdgcn_inverse_ballot_w64(takem);
-            const float r2 = sel ? d2e : 1.f;  // finite on every lane
+            const float r2 = SOFT ? d2e : (sel ? d2e : 1.f);  // finite on every lane (d2e >= eps^2 > 0 when SOFT)
             const float rs = __builtin_amdgcn_rsqf(r2);
             const float rs2 = rs * rs;
             const float s3 = sel ? rs * rs2 : 0.f;
@@ -257,49 +258,69 @@ __global__ void log_count_kernel(DeviceState *st, int64_t cap)
     }
 }
 
-// Per-node sums of the sorted log, one wave per 64 sorted records (one per
-// lane, gathered from the log in one instruction): an inclusive scan over the
-// lanes holding the same node (six shuffle steps), then the last lane of each
-// run adds it to the node.  A run that continues into the previous or the next
-// 64 records is added with an FP64 atomic; a run inside the 64 with a plain add
-// (no other wave holds that node).
-__global__ void __launch_bounds__(256) log_reduce_kernel(const uint64_t *__restrict__ sk,
+// Per-node sums of the sorted log in tiles of 64 records (one per lane); a wave
+// takes kReduceTiles consecutive tiles and issues all their loads (ids, then the
+// values gathered from the log) before the first sum.  Per tile: an inclusive
+// scan over the lanes holding the same node (six shuffle steps), then the last
+// lane of each run adds it to the node.  A run that continues into the previous
+// or the next tile is added with an FP64 atomic; a run inside the tile with a
+// plain add (no other tile holds that node).
+constexpr int kReduceTiles = 4;
+
+__global__ void __launch_bounds__(256) log_reduce_kernel(const uint32_t *__restrict__ sk,
                                                          const int32_t *__restrict__ sv,
                                                          const float4 *__restrict__ val, const DeviceState *st,
-                                                         uint64_t sentinel, double *__restrict__ tgt)
+                                                         uint32_t sentinel, double *__restrict__ tgt)
 {
     if (latched(st)) return;
     const int64_t count = st->log_count;
     const int lane = threadIdx.x & 63;
     const int64_t ntiles = (count + 63) / 64;
-    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < ntiles; w += wstride) {
-        const int64_t k0 = w * 64, k = k0 + lane;
-        const uint64_t id = k < count ? sk[k] : sentinel;
-        const bool live = id != sentinel;
-        float4 v = live ? val[sv[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const int id32 = live ? (int)id : -1;
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x >> 6) * kReduceTiles;
+    for (int64_t w0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kReduceTiles; w0 < ntiles;
+         w0 += wstride) {
+        uint32_t ids[kReduceTiles];
+        int32_t src[kReduceTiles];
+        float4 vals[kReduceTiles];
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int oid = __shfl_up(id32, o);
-            const float tx = __shfl_up(v.x, o), ty = __shfl_up(v.y, o), tz = __shfl_up(v.z, o),
-                        tw = __shfl_up(v.w, o);
-            if (lane >= o && oid == id32) {
-                v.x += tx; v.y += ty; v.z += tz; v.w += tw;
-            }
+        for (int t = 0; t < kReduceTiles; ++t) {
+            const int64_t k = (w0 + t) * 64 + lane;
+            ids[t] = k < count ? sk[k] : sentinel;
+            src[t] = k < count ? sv[k] : 0;
         }
-        const int next = __shfl_down(id32, 1);
-        const bool last_of_run = live && (lane == 63 || next != id32);
-        const int lane0_id = __shfl(id32, 0);
-        const bool from_prev = k0 > 0 && lane0_id == id32 && sk[k0 - 1] == id;
-        const bool into_next = lane == 63 && k + 1 < count && sk[k + 1] == id;
-        if (last_of_run) {
-            double *p = tgt + 4 * (int64_t)id;
-            if (from_prev || into_next) {
-                unsafeAtomicAdd(p + 0, (double)v.x); unsafeAtomicAdd(p + 1, (double)v.y);
-                unsafeAtomicAdd(p + 2, (double)v.z); unsafeAtomicAdd(p + 3, (double)v.w);
-            } else {
-                p[0] += v.x; p[1] += v.y; p[2] += v.z; p[3] += v.w;
+#pragma unroll
+        for (int t = 0; t < kReduceTiles; ++t)
+            vals[t] = ids[t] != sentinel ? val[src[t]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int t = 0; t < kReduceTiles; ++t) {
+            if (w0 + t >= ntiles) break;
+            const int64_t k0 = (w0 + t) * 64, k = k0 + lane;
+            const uint32_t id = ids[t];
+            const bool live = id != sentinel;
+            float4 v = vals[t];
+            const int id32 = live ? (int)id : -1;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int oid = __shfl_up(id32, o);
+                const float tx = __shfl_up(v.x, o), ty = __shfl_up(v.y, o), tz = __shfl_up(v.z, o),
+                            tw = __shfl_up(v.w, o);
+                if (lane >= o && oid == id32) {
+                    v.x += tx; v.y += ty; v.z += tz; v.w += tw;
+                }
+            }
+            const int next = __shfl_down(id32, 1);
+            const bool last_of_run = live && (lane == 63 || next != id32);
+            const int lane0_id = __shfl(id32, 0);
+            const bool from_prev = k0 > 0 && lane0_id == id32 && sk[k0 - 1] == id;
+            const bool into_next = lane == 63 && k + 1 < count && sk[k + 1] == id;
+            if (last_of_run) {
+                double *p = tgt + 4 * (int64_t)id;
+                if (from_prev || into_next) {
+                    unsafeAtomicAdd(p + 0, (double)v.x); unsafeAtomicAdd(p + 1, (double)v.y);
+                    unsafeAtomicAdd(p + 2, (double)v.z); unsafeAtomicAdd(p + 3, (double)v.w);
+                } else {
+                    p[0] += v.x; p[1] += v.y; p[2] += v.z; p[3] += v.w;
+                }
             }
         }
     }
@@ -427,19 +448,20 @@ void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy,
     clear_f64_kernel<<<(int)std::min<int64_t>((ntgt + 255) / 256, 65536), 256, 0, s>>>(c->adj_tgt, ntgt);
     vjp_reset_kernel<<<1, 1, 0, s>>>(c->st);
     const int passes = sort_passes(c->m_max);
-    const uint64_t sentinel = (uint64_t(1) << (kRadixBits * passes)) - 1;
+    const uint32_t sentinel = (uint32_t)((uint64_t(1) << (kRadixBits * passes)) - 1);
     constexpr int W = 2;
     const int ngroups = (n + 63) / 64;
-    vjp_kernel<W><<<(ngroups + W - 1) / W, 64 * W, 0, s>>>(
-        n, ngroups, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->perm, c->gorder, c->st,
-        layout_eps2(c->p), g, ux, uy, uz, gx, gy, gz, c->adj_tgt, c->adj_log_id, c->adj_log_val,
-        c->adj_cap, sentinel, c->st, &c->st->err);
+    const float eps2 = layout_eps2(c->p);
+    auto walk = eps2 > 0.f ? vjp_kernel<W, true> : vjp_kernel<W, false>;
+    walk<<<(ngroups + W - 1) / W, 64 * W, 0, s>>>(
+        n, ngroups, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->perm, c->gorder, c->st, eps2, g, ux, uy,
+        uz, gx, gy, gz, c->adj_tgt, c->adj_log_id, c->adj_log_val, c->adj_cap, sentinel, c->st, &c->st->err);
     log_count_kernel<<<1, 1, 0, s>>>(c->st, c->adj_cap);
-    uint64_t *tk = passes % 2 ? c->adj_log_id : c->adj_tk;  // the input keys are dead after the first pass
+    uint32_t *tk = passes % 2 ? c->adj_log_id : c->adj_tk;  // the input keys are dead after the first pass
     launch_radix_passes(c->adj_hist, s, c->adj_log_id, nullptr, (int)c->adj_cap, 0, passes, c->adj_sk, c->adj_sv, tk,
                         c->adj_tv, nullptr, &c->st->log_count);
-    const int64_t tiles = (c->adj_cap + 63) / 64;  // one wave per 64 sorted records
-    log_reduce_kernel<<<(int)std::min<int64_t>((tiles + 3) / 4, 65536), 256, 0, s>>>(
+    const int64_t waves = (c->adj_cap + 64 * kReduceTiles - 1) / (64 * kReduceTiles);
+    log_reduce_kernel<<<(int)std::min<int64_t>((waves + 3) / 4, 65536), 256, 0, s>>>(
         c->adj_sk, c->adj_sv, c->adj_log_val, c->st, sentinel, c->adj_tgt);
     const int64_t nblk = c->m - n + 1;
     adj_cells_kernel<<<(int)std::min<int64_t>((nblk + 255) / 256, 65536), 256, 0, s>>>(

@@ -1,12 +1,13 @@
-// k_radix.hip -- stable LSD radix sort of (64-bit key, int32 index) pairs.
+// k_radix.hip -- stable LSD radix sort of (64- or 32-bit key, int32 index) pairs.
 //
 // Passes of kRadixBits = 8 bits.  Each pass is three launches over tiles of 2048
 // keys (4 waves x 8 items x 64 lanes; wave w owns the contiguous quarter w of the
 // tile, walked item by item so that lane order is element order):
-//   upsweep   per-tile digit histogram           reads 8 B/key
+//   upsweep   per-tile digit histogram           reads 8 B/key (64-bit keys)
 //   scan      exclusive scan of each digit row       ~1 KiB per tile
 //   downsweep stable rank, the tile reordered by digit in LDS, then written out
 //             digit run by digit run (coalesced)  reads 12 B/key, writes 12 B/key
+// (32-bit keys: 4 and 8 B/key).
 // Ranks inside a wave come from a 64-lane match of the digit built from
 // kRadixBits ballots (no atomics, no intra-wave waits), so equal digits keep
 // their order.
@@ -29,12 +30,14 @@ __device__ inline uint64_t match_digit(uint32_t digit, uint64_t live)
     return peers;
 }
 
+template <class K>
 __global__ void __launch_bounds__(kSortThreads) radix_upsweep(
-    const uint64_t *__restrict__ keys, int n, int shift, int ntiles, uint32_t *__restrict__ hist,
+    const K *__restrict__ keys, int n, int shift, int ntiles, uint32_t *__restrict__ hist,
     const int *__restrict__ n_dev)
 {
     __shared__ uint32_t cnt[kSortThreads / 64][kRadix];
     if (n_dev) n = min(n, *n_dev);  // keys beyond the device-side count take no part
+    if ((int64_t)blockIdx.x * kSortTile >= n) return;  // a tile past the count: not scanned, not read
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int k = tid; k < (kSortThreads / 64) * kRadix; k += kSortThreads) (&cnt[0][0])[k] = 0u;
     __syncthreads();
@@ -77,14 +80,17 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh, uint32
 
 // Exclusive scan of every digit row hist[d][0..ntiles) in place (one block per
 // digit); row totals go to rowsum[d].  The digit bases (exclusive scan of the
-// row totals) are formed by each downsweep block.
+// row totals) are formed by each downsweep block.  With a device-side count only
+// the tiles that hold keys are scanned.
 __global__ void __launch_bounds__(256) radix_rows_scan(uint32_t *__restrict__ hist, int ntiles,
-                                                       uint32_t *__restrict__ rowsum)
+                                                       uint32_t *__restrict__ rowsum, int n,
+                                                       const int *__restrict__ n_dev)
 {
     __shared__ uint32_t sh[256];
     uint32_t *row = hist + (int64_t)blockIdx.x * ntiles;
-    const int per = (ntiles + 255) / 256;
-    const int lo = threadIdx.x * per, hi = min(ntiles, lo + per);
+    const int used = n_dev ? (int)min((int64_t)ntiles, ((int64_t)min(n, *n_dev) + kSortTile - 1) / kSortTile) : ntiles;
+    const int per = (used + 255) / 256;
+    const int lo = threadIdx.x * per, hi = min(used, lo + per);
     uint32_t s = 0;
     for (int k = lo; k < hi; ++k) s += row[k];
     uint32_t total;
@@ -136,18 +142,20 @@ __global__ void __launch_bounds__(256) scan_add_kernel(uint32_t *__restrict__ da
         if (base + j < total) data[base + j] += off;
 }
 
+template <class K>
 __global__ void __launch_bounds__(kSortThreads) radix_downsweep(
-    const uint64_t *__restrict__ kin, const int32_t *__restrict__ vin, int n, int shift, int ntiles,
-    const uint32_t *__restrict__ hist, const uint32_t *__restrict__ rowsum, uint64_t *__restrict__ kout,
+    const K *__restrict__ kin, const int32_t *__restrict__ vin, int n, int shift, int ntiles,
+    const uint32_t *__restrict__ hist, const uint32_t *__restrict__ rowsum, K *__restrict__ kout,
     int32_t *__restrict__ vout, int32_t *__restrict__ inv, const int *__restrict__ n_dev)
 {
     if (n_dev) n = min(n, *n_dev);
+    if ((int64_t)blockIdx.x * kSortTile >= n) return;  // no keys in this tile
     constexpr int W = kSortThreads / 64;
     __shared__ uint32_t run[W][kRadix];
     __shared__ uint32_t dbase[kRadix];
     __shared__ uint32_t gdelta[kRadix];
     __shared__ uint32_t scan_sh[kSortThreads];
-    __shared__ uint64_t s_key[kSortTile];
+    __shared__ K s_key[kSortTile];
     __shared__ int32_t s_val[kSortTile];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int k = tid; k < W * kRadix; k += kSortThreads) (&run[0][0])[k] = 0u;
@@ -171,7 +179,7 @@ __global__ void __launch_bounds__(kSortThreads) radix_downsweep(
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kSortTile + (int64_t)wave * (kSortItems * 64);
     const uint64_t lt = lanemask_lt(lane);
-    uint64_t key[kSortItems];
+    K key[kSortItems];
     int32_t val[kSortItems];
     uint32_t rank[kSortItems];
     uint32_t dig[kSortItems];
@@ -179,7 +187,7 @@ __global__ void __launch_bounds__(kSortThreads) radix_downsweep(
     for (int i = 0; i < kSortItems; ++i) {
         const int64_t e = base + i * 64 + lane;
         const bool valid = e < n;
-        key[i] = valid ? kin[e] : 0ull;
+        key[i] = valid ? kin[e] : K(0);
         val[i] = valid ? (vin ? vin[e] : (int32_t)e) : 0;
         const uint32_t d = (uint32_t)((key[i] >> shift) & (kRadix - 1));
         dig[i] = d;
@@ -239,7 +247,7 @@ __global__ void __launch_bounds__(kSortThreads) radix_downsweep(
     const int64_t t0 = (int64_t)blockIdx.x * kSortTile;
     const int tn = (int)min((int64_t)kSortTile, (int64_t)n - t0);
     for (int j = tid; j < tn; j += kSortThreads) {
-        const uint64_t k = s_key[j];
+        const K k = s_key[j];
         const int32_t v = s_val[j];
         const uint32_t d = (uint32_t)((k >> shift) & (kRadix - 1));
         const uint32_t p = (uint32_t)j + gdelta[d];
@@ -249,30 +257,36 @@ __global__ void __launch_bounds__(kSortThreads) radix_downsweep(
     }
 }
 
-void launch_radix_passes(uint32_t *hist, hipStream_t s, const uint64_t *keys_in, const int32_t *vals_in, int n,
-                         int first_shift, int npasses, uint64_t *keys_out, int32_t *vals_out, uint64_t *tmp_k,
+template <class K>
+void launch_radix_passes(uint32_t *hist, hipStream_t s, const K *keys_in, const int32_t *vals_in, int n,
+                         int first_shift, int npasses, K *keys_out, int32_t *vals_out, K *tmp_k,
                          int32_t *tmp_v, int32_t *inv_out, const int *n_dev)
 {
     if (n <= 0 || npasses <= 0) return;
     const int ntiles = (n + kSortTile - 1) / kSortTile;
-    const uint64_t *ki = keys_in;
+    const K *ki = keys_in;
     const int32_t *vi = vals_in;
     for (int p = 0; p < npasses; ++p) {
         const int shift = first_shift + p * kRadixBits;
         const bool last = (p == npasses - 1);
         // alternate so that the last pass lands in keys_out / vals_out
         const bool to_out = ((npasses - 1 - p) % 2) == 0;
-        uint64_t *ko = to_out ? keys_out : tmp_k;
+        K *ko = to_out ? keys_out : tmp_k;
         int32_t *vo = to_out ? vals_out : tmp_v;
         uint32_t *rowsum = hist + (int64_t)ntiles * kRadix;  // kRadix totals after the rows
-        radix_upsweep<<<ntiles, kSortThreads, 0, s>>>(ki, n, shift, ntiles, hist, n_dev);
-        radix_rows_scan<<<kRadix, 256, 0, s>>>(hist, ntiles, rowsum);
-        radix_downsweep<<<ntiles, kSortThreads, 0, s>>>(ki, vi, n, shift, ntiles, hist, rowsum, ko, vo,
+        radix_upsweep<K><<<ntiles, kSortThreads, 0, s>>>(ki, n, shift, ntiles, hist, n_dev);
+        radix_rows_scan<<<kRadix, 256, 0, s>>>(hist, ntiles, rowsum, n, n_dev);
+        radix_downsweep<K><<<ntiles, kSortThreads, 0, s>>>(ki, vi, n, shift, ntiles, hist, rowsum, ko, vo,
                                                         last ? inv_out : nullptr, n_dev);
         ki = ko;
         vi = vo;
     }
 }
+
+template void launch_radix_passes<uint64_t>(uint32_t *, hipStream_t, const uint64_t *, const int32_t *, int, int, int,
+                                            uint64_t *, int32_t *, uint64_t *, int32_t *, int32_t *, const int *);
+template void launch_radix_passes<uint32_t>(uint32_t *, hipStream_t, const uint32_t *, const int32_t *, int, int, int,
+                                            uint32_t *, int32_t *, uint32_t *, int32_t *, int32_t *, const int *);
 
 // Batched form: after the 8 key passes, a stable pass over the system index
 // groups every system's bodies, in key order, at [off[s], off[s+1]).

@@ -102,7 +102,7 @@ struct nbx_context {
     double *adj_ab = nullptr;    // [4 (m_max + 1)] per block: sum over ancestors of G/M and GM - G.X/M
     // node-term log: one record {node, float4} per child step with accepting lanes
     int64_t adj_cap = 0;         // records
-    uint64_t *adj_log_id = nullptr, *adj_sk = nullptr, *adj_tk = nullptr;
+    uint32_t *adj_log_id = nullptr, *adj_sk = nullptr, *adj_tk = nullptr;
     float4 *adj_log_val = nullptr;
     int32_t *adj_sv = nullptr, *adj_tv = nullptr;
     uint32_t *adj_hist = nullptr;
@@ -198,9 +198,11 @@ constexpr int kMortonSortBits = 63;
 void launch_exclusive_scan_u32(uint32_t *data, int total, uint32_t *sums, hipStream_t s);
 // npasses passes of kRadixBits bits from bit first_shift; vals_in == nullptr means 0, 1, 2, ...
 // n_dev (nullable): device-side count; only the first min(n, *n_dev) keys are sorted
-// (grids are sized for n).  tmp_k may alias keys_in when npasses is odd.
-void launch_radix_passes(uint32_t *hist, hipStream_t s, const uint64_t *keys_in, const int32_t *vals_in, int n,
-                         int first_shift, int npasses, uint64_t *keys_out, int32_t *vals_out, uint64_t *tmp_k,
+// (grids are sized for n).  tmp_k may alias keys_in when npasses is odd.  K: uint64_t
+// or uint32_t.
+template <class K>
+void launch_radix_passes(uint32_t *hist, hipStream_t s, const K *keys_in, const int32_t *vals_in, int n,
+                         int first_shift, int npasses, K *keys_out, int32_t *vals_out, K *tmp_k,
                          int32_t *tmp_v, int32_t *inv_out, const int *n_dev = nullptr);
 
 // launchers (k_tree.hip)
