@@ -872,17 +872,19 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
 // cells far apart), so bodies are ordered by the top kGroupLevels Hilbert digits
 // of their key instead, stably over key order (ties keep key order); the
 // ensemble puts the system index above them.  The order changes which bodies
-// share a wave, never a body's interaction list.
+// share a wave, never a body's interaction list.  A single system's keys
+// (3 kGroupLevels = 30 bits) are sorted as 32-bit keys.
+template <class K>
 __global__ void __launch_bounds__(256) group_keys_kernel(int n, const uint64_t *__restrict__ skeys,
                                                          const int32_t *__restrict__ perm,
                                                          const int32_t *__restrict__ sys_of,
-                                                         uint64_t *__restrict__ gkey, const DeviceState *st)
+                                                         K *__restrict__ gkey, const DeviceState *st)
 {
     if (latched(st)) return;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += blockDim.x * gridDim.x) {
         uint64_t h = hilbert_top(skeys[k] & ~(1ull << 63), kGroupLevels);
         if (sys_of) h |= (uint64_t)sys_of[perm[k]] << (3 * kGroupLevels);
-        gkey[k] = h;
+        gkey[k] = (K)h;
     }
 }
 
@@ -976,9 +978,10 @@ void launch_layout(nbx_context *c, hipStream_t s, int *sorted)
     const int n = c->n;
     sorted_index_kernel<<<capped_grid(n), 256, 0, s>>>(n, c->perm, sorted, c->st);
     launch_frame_layout(c, s);
-    group_keys_kernel<<<capped_grid(n), 256, 0, s>>>(n, c->skeys, c->perm, nullptr, c->gkey, c->st);
-    launch_radix_passes(c->hist, s, c->gkey, nullptr, n, 0, group_passes(0), c->gkey2, c->gorder, c->kbuf, c->vbuf,
-                        nullptr);
+    static_assert(3 * kGroupLevels <= 32, "single-system group keys are 32-bit");
+    group_keys_kernel<uint32_t><<<capped_grid(n), 256, 0, s>>>(n, c->skeys, c->perm, nullptr, c->gkey, c->st);
+    launch_radix_passes(c->hist, s, c->gkey, nullptr, n, 0, group_passes(0), c->gkey2, c->gorder,
+                        reinterpret_cast<uint32_t *>(c->kbuf), c->vbuf, nullptr);
 }
 
 void launch_frame_layout(nbx_context *c, hipStream_t s)
@@ -1016,7 +1019,7 @@ void launch_batch_layout(nbx_batch *b, hipStream_t s)
         b->frootm, b->pblk, b->bpar, 0.f, layout_eps2(b->p), si, b->st);
     int sys_bits = 0;
     while ((1 << sys_bits) < b->S) ++sys_bits;
-    group_keys_kernel<<<capped_grid(b->N), 256, 0, s>>>(b->N, b->skeys, b->perm, b->sys_of, b->gkey, b->st);
+    group_keys_kernel<uint64_t><<<capped_grid(b->N), 256, 0, s>>>(b->N, b->skeys, b->perm, b->sys_of, b->gkey, b->st);
     launch_radix_passes(b->hist, s, b->gkey, nullptr, b->N, 0, group_passes(sys_bits), b->gkey2, b->gorder, b->kbuf,
                         b->vbuf, nullptr);
 }

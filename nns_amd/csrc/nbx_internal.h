@@ -111,7 +111,7 @@ struct nbx_context {
     int64_t *counts = nullptr;
     float *acc_scratch = nullptr;
     float4 *bodies4 = nullptr;  // (x, y, z, m) of every body in input order
-    uint64_t *gkey = nullptr, *gkey2 = nullptr;  // group-order sort keys
+    uint32_t *gkey = nullptr, *gkey2 = nullptr;  // group-order sort keys
     int32_t *gorder = nullptr;  // [n] sorted position of the j-th body of the force groups
     nbx::TreeScratch ts;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

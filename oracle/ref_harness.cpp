@@ -26,10 +26,16 @@
 //       order (Interaction.cpp:179-186) and Interaction::treeInteraction per body
 //       (:203-208) with TIME_STEP = 1 and SOFTENING = softening_m.  dv is the
 //       velocity increment the reference accumulates in Body::acceleration.
-//   time IN theta softening_m nsample
+//   time IN theta softening_m nsample [hashed]
 //       IN : int64 n, float x[n], y[n], z[n], m[n]
 //       Times the serial Tree::insert of all n bodies and the OpenMP
 //       treeInteraction loop over every (n/nsample)-th body; prints one JSON line.
+//       hashed = 1 walks the bodies in key order, the reference's `hashed` path
+//       (Interaction.cpp:156-177,190-197): Body::discretizePosition +
+//       Body::getKey per body into a std::map<key, Body*> (a repeated key keeps
+//       the last body), and the sample is every (n/nsample)-th body of that
+//       order.  The map is timed on its own (key_s), as the reference leaves it
+//       outside its build and force timers.
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -172,7 +178,7 @@ int cmd_tree(const char *in, const char *out, double theta, double soft, double 
     return 0;
 }
 
-int cmd_time(const char *in, double theta, double soft, int64_t nsample)
+int cmd_time(const char *in, double theta, double soft, int64_t nsample, int hashed)
 {
     FILE *f = fopen(in, "rb");
     if (!f) return 2;
@@ -195,6 +201,23 @@ int cmd_time(const char *in, double theta, double soft, int64_t nsample)
         bods[i].mass = mf[i];
         bods[i].position = Vector3D(x[i], y[i], z[i]);
     }
+    std::vector<Body *> order;  // traversal order
+    double key_s = 0.0;
+    if (hashed) {
+        auto k0 = std::chrono::steady_clock::now();
+        std::map<uint64_t, Body *> bodies;
+        for (int64_t i = 0; i < n; ++i) {
+            bods[i].discretizePosition(s.span, Vector3D(s.x0, s.y0, s.z0));
+            bodies[bods[i].getKey()] = &bods[i];
+        }
+        order.reserve(bodies.size());
+        for (auto const &b : bodies) order.push_back(b.second);
+        key_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - k0).count();
+    } else {
+        order.reserve(n);
+        for (int64_t i = 0; i < n; ++i) order.push_back(&bods[i]);
+    }
+    const int64_t norder = (int64_t)order.size();
     auto t0 = std::chrono::steady_clock::now();
     Tree *tree = new Tree(Octant(cx, cy, cz, len));
     int64_t inserted = 0;
@@ -204,18 +227,20 @@ int cmd_time(const char *in, double theta, double soft, int64_t nsample)
     if (nsample <= 0 || nsample > n) nsample = n;
     int64_t stride = n / nsample;
     if (stride < 1) stride = 1;
-    int64_t ns = (n + stride - 1) / stride;
+    int64_t ns = (norder + stride - 1) / stride;
     #pragma omp parallel for
     for (int64_t k = 0; k < ns; ++k) {
-        Body *b = &bods[k * stride];
+        Body *b = order[k * stride];
         if (tree->getOctant().contains(b->position)) inter.treeInteraction(tree, b);
     }
     auto t2 = std::chrono::steady_clock::now();
     double build_s = std::chrono::duration<double>(t1 - t0).count();
     double force_s = std::chrono::duration<double>(t2 - t1).count();
     printf("{\"n\": %lld, \"inserted\": %lld, \"build_s\": %.6f, \"force_sample_s\": %.6f, "
-           "\"nsample\": %lld, \"threads\": %d, \"theta\": %g}\n",
-           (long long)n, (long long)inserted, build_s, force_s, (long long)ns, omp_get_max_threads(), theta);
+           "\"nsample\": %lld, \"threads\": %d, \"theta\": %g, \"order\": \"%s\", \"ordered\": %lld, "
+           "\"key_s\": %.6f}\n",
+           (long long)n, (long long)inserted, build_s, force_s, (long long)ns, omp_get_max_threads(), theta,
+           hashed ? "key" : "index", (long long)norder, key_s);
     delete tree;
     delete[] bods;
     return 0;
@@ -234,7 +259,8 @@ int main(int argc, char **argv)
     if (c == "tree" && argc == 10)
         return cmd_tree(argv[2], argv[3], atof(argv[4]), atof(argv[5]), atof(argv[6]), atof(argv[7]),
                         atof(argv[8]), atof(argv[9]));
-    if (c == "time" && argc == 6) return cmd_time(argv[2], atof(argv[3]), atof(argv[4]), atoll(argv[5]));
+    if (c == "time" && (argc == 6 || argc == 7))
+        return cmd_time(argv[2], atof(argv[3]), atof(argv[4]), atoll(argv[5]), argc == 7 ? atoi(argv[6]) : 0);
     fprintf(stderr, "bad arguments\n");
     return 1;
 }
