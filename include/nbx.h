@@ -75,7 +75,8 @@ typedef struct nbx_params {
     float timestep;   /* dt                                                             */
     float dampening;  /* d in x += d dt v                                               */
     int kahan;        /* 1: compensated (Kahan) accumulation of the accelerations       */
-    int block_size;   /* threads per block of the streaming stages (0 = 256)            */
+    int block_size;   /* threads per block of the update stage: 0 (= 256) or a multiple  */
+                      /* of 64 in [64, 1024] (the reference's blockSize, main.cu:30)   */
     int force_waves;  /* waves per block of the force kernel (0 = 4)                    */
     int build_locked; /* 1: subtrees built by lock-based insertion (compare-and-swap on
                          child slots, Kernels.cu:138-241); 0: from the shared-digit

@@ -56,7 +56,7 @@ def build(runner: bool = False, verbose: bool = False) -> str:
     if runner:
         cmd = [HIPCC, *COMMON, "-x", "hip", os.path.join(CSRC, "nbx_runner.cpp"),
                os.path.join(CSRC, "barnes_hut.cpp"), "-o", RUNNER, f"-L{HERE}", "-lnbx",
-               f"-Wl,-rpath,{HERE}"]
+               "-Wl,-rpath,$ORIGIN"]  # libnbx.so beside the runner, wherever the tree is
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"runner build failed:\n{r.stdout}\n{r.stderr}")

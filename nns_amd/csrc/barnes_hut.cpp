@@ -63,6 +63,7 @@ BarnesHut::BarnesHut(const SimulationParameters &p) : parameters(p)
     np.dampening = p.dampening;
     np.kahan = p.kahan ? 1 : 0;
     np.build_locked = p.buildLocked ? 1 : 0;
+    np.block_size = p.blockSize;
     check(nbx_create(&np, numParticles, numNodes, 0, &ctx), "nbx_create");
 }
 

@@ -25,6 +25,17 @@ struct SimulationParameters {  // include/Constants.h:4-22 plus the engine's kno
     std::string model = "binary_disk";  // star_disk | plummer | binary_disk
     unsigned long long seed = 1;
     bool copyToHost = false;            // the reference copies x, v back every step
+    // The reference's launch and rendering settings (include/Constants.h:15-19).
+    // blockSize sets the threads per block of the streaming stages; gridSize,
+    // warp and stackSize are kept for the interface only (grids are sized from
+    // N, a walk is one 64-lane wave with its stack in LDS), and nothing is
+    // rendered (renderInterval, render).
+    int gridSize = 1024;
+    int blockSize = 256;
+    int warp = 32;
+    int stackSize = 64;
+    int renderInterval = 10;
+    bool render = false;
 };
 
 class BarnesHut {

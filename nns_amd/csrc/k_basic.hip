@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(256) keys_kernel(
 
 // updateKernel (Kernels.cu:581-600): v += dt a; x += d dt v, with explicit fused
 // multiply-adds so that the result does not depend on compiler contraction.
-__global__ void __launch_bounds__(256) update_kernel(
+__global__ void __launch_bounds__(1024) update_kernel(
     float *__restrict__ x, float *__restrict__ y, float *__restrict__ z, float *__restrict__ vx,
     float *__restrict__ vy, float *__restrict__ vz, const float *__restrict__ ax,
     const float *__restrict__ ay, const float *__restrict__ az, int n, float dt, float d)

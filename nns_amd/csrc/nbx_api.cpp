@@ -41,7 +41,8 @@ int hip_fail(nbx_context *c, hipError_t e, const char *what)
 bool params_ok(const nbx_params *p)
 {
     return p && p->theta >= 0.f && p->softening >= 0.f && p->timestep == p->timestep &&
-           p->gravity == p->gravity && p->dampening == p->dampening;
+           p->gravity == p->gravity && p->dampening == p->dampening &&
+           (p->block_size == 0 || (p->block_size >= 64 && p->block_size <= 1024 && p->block_size % 64 == 0));
 }
 
 struct StageTimer {

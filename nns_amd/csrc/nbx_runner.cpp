@@ -1,9 +1,13 @@
 // nbx_runner.cpp -- command-line runner, the counterpart of the reference's
-// ./bin/runner (3D/CUDA/CUDA_NBody/src/main.cu:19-190): same options -n -i -v,
-// per-stage [TIME] lines per step and the per-stage totals at the end.
+// ./bin/runner (3D/CUDA/CUDA_NBody/src/main.cu:19-190): the same options
+// (-r -i -n -b -g -R -v -h, with the same defaults), per-stage [TIME] lines per
+// step and the per-stage totals at the end.  -b sets the threads per block of
+// the streaming stages; -g, -R and -r are accepted and have no effect (grids
+// are sized from N; nothing is rendered).
 //
-//   nbx_runner [-n particles] [-i iterations] [-v] [--model binary_disk|plummer|star_disk]
-//              [--theta T] [--softening E] [--seed S] [--copy] [--locked-build]
+//   nbx_runner [-n particles] [-i iterations] [-b blocksize] [-g gridsize] [-R interval] [-r] [-v]
+//              [--model binary_disk|plummer|star_disk] [--theta T] [--softening E] [--seed S]
+//              [--copy] [--locked-build]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -27,6 +31,10 @@ int main(int argc, char **argv)
         };
         if (a == "-n" || a == "--particles") p.numberOfParticles = std::atoi(next("-n"));
         else if (a == "-i" || a == "--iterations") p.iterations = std::atoi(next("-i"));
+        else if (a == "-b" || a == "--blocksize") p.blockSize = std::atoi(next("-b"));
+        else if (a == "-g" || a == "--gridsize") p.gridSize = std::atoi(next("-g"));
+        else if (a == "-R" || a == "--renderinterval") p.renderInterval = std::atoi(next("-R"));
+        else if (a == "-r" || a == "--render") p.render = true;
         else if (a == "-v" || a == "--verbosity") verbose = true;
         else if (a == "--model") p.model = next("--model");
         else if (a == "--theta") p.theta = (float)std::atof(next("--theta"));
@@ -35,16 +43,18 @@ int main(int argc, char **argv)
         else if (a == "--copy") p.copyToHost = true;
         else if (a == "--locked-build") p.buildLocked = true;
         else if (a == "-h" || a == "--help") {
-            std::printf("nbx_runner [-n particles] [-i iterations] [-v] [--model binary_disk|plummer|star_disk]\n"
-                        "           [--theta T] [--softening E] [--seed S] [--copy] [--locked-build]\n");
+            std::printf("nbx_runner [-n particles] [-i iterations] [-b blocksize] [-g gridsize] [-R interval] [-r]\n"
+                        "           [-v] [--model binary_disk|plummer|star_disk] [--theta T] [--softening E]\n"
+                        "           [--seed S] [--copy] [--locked-build]\n");
             return 0;
         } else {
             std::fprintf(stderr, "unknown option %s\n", a.c_str());
             return 1;
         }
     }
-    if (p.numberOfParticles < 2 || p.iterations < 1) {
-        std::fprintf(stderr, "need -n >= 2 and -i >= 1\n");
+    if (p.numberOfParticles < 2 || p.iterations < 1 || p.blockSize < 64 || p.blockSize > 1024 ||
+        p.blockSize % 64 != 0) {
+        std::fprintf(stderr, "need -n >= 2, -i >= 1 and -b a multiple of 64 in [64, 1024]\n");
         return 1;
     }
     try {

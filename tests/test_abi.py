@@ -44,6 +44,10 @@ def test_create_validates_before_device():
     assert L.nbx_create(C.byref(p), 100, 100, 0, C.byref(h)) == -1
     bad = nns_amd.default_params(theta=-1.0)
     assert L.nbx_create(C.byref(bad), 100, 1000, 0, C.byref(h)) == -1
+    for bs in (100, 32, 2048):  # block_size: 0 or a multiple of 64 in [64, 1024]
+        bad = nns_amd.default_params(block_size=bs)
+        assert L.nbx_create(C.byref(bad), 100, 1000, 0, C.byref(h)) == -1
+        assert b"invalid parameters" in L.nbx_last_error(None)
     assert not h.value
 
 
@@ -109,3 +113,15 @@ def test_star_disk_layout():
     assert r.min() >= 0.1 * (1 - 1e-6) and r.max() <= 1.0 * (1 + 1e-6)
     assert np.std(z[1:]) == pytest.approx(0.01, rel=0.05)
     assert np.all(m[1:] == np.float32(1e-6))
+
+
+def test_runner_rejects_bad_options():
+    """nns_amd/nbx_runner checks its options before it touches the GPU."""
+    import subprocess
+    runner = os.path.join(os.path.dirname(nns_amd.__file__), "nbx_runner")
+    assert os.access(runner, os.X_OK), "nns_amd/nbx_runner is not built (python -m nns_amd.build --runner)"
+    for args in (["-b", "100"], ["-b", "2048"], ["-n", "1"], ["-i", "0"], ["--no-such-option"], ["-n"]):
+        r = subprocess.run([runner, *args], capture_output=True, text=True, timeout=60)
+        assert r.returncode != 0, args
+    r = subprocess.run([runner, "-h"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "-b blocksize" in r.stdout and "-R interval" in r.stdout
