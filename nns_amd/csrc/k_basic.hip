@@ -11,7 +11,7 @@ __global__ void __launch_bounds__(256) reset_kernel(
     int *__restrict__ mutex, float *__restrict__ x, float *__restrict__ y, float *__restrict__ z,
     float *__restrict__ mass, int *__restrict__ count, int *__restrict__ start, int *__restrict__ sorted,
     int *__restrict__ child, int *__restrict__ index, float *minX, float *maxX, float *minY, float *maxY,
-    float *minZ, float *maxZ, int n, int m, DeviceState *st, int *arrive)
+    float *minZ, float *maxZ, int n, int m, DeviceState *st)
 {
     const int stride = blockDim.x * gridDim.x;
     const int4 neg = make_int4(-1, -1, -1, -1);
@@ -34,7 +34,6 @@ __global__ void __launch_bounds__(256) reset_kernel(
         *minX = 0.f; *maxX = 0.f; *minY = 0.f; *maxY = 0.f; *minZ = 0.f; *maxZ = 0.f;
         st->err = 0;
         st->bbox_ticket = 0u;
-        arrive[0] = 0;
     }
 }
 
@@ -188,7 +187,7 @@ void launch_reset(nbx_context *c, hipStream_t s, int *mutex, float *x, float *y,
     const int block = 256;
     reset_kernel<<<grid_for(m, block, 8192), block, 0, s>>>(mutex, x, y, z, mass, count, start, sorted, child,
                                                             index, minX, maxX, minY, maxY, minZ, maxZ, n, m,
-                                                            c->st, c->arrive);
+                                                            c->st);
 }
 
 void launch_bbox(nbx_context *c, hipStream_t s, const float *x, const float *y, const float *z,
@@ -220,11 +219,9 @@ void launch_fill_f32(hipStream_t s, float *p, float v, int64_t n)
 }
 
 // ------------------------------------------------------------ batched form
-// Root cell N + s of every system: empty slots, level 0, no parent.
+// Root cell N + s of every system: empty slots, level 0.
 __global__ void __launch_bounds__(256) reset_batch_kernel(int S, int N, int *__restrict__ child, int *index,
-                                                          int32_t *__restrict__ cell_level,
-                                                          int32_t *__restrict__ parent,
-                                                          int32_t *__restrict__ arrive, DeviceState *st)
+                                                          int32_t *__restrict__ cell_level, DeviceState *st)
 {
     const int4 neg = make_int4(-1, -1, -1, -1);
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += blockDim.x * gridDim.x) {
@@ -233,8 +230,6 @@ __global__ void __launch_bounds__(256) reset_batch_kernel(int S, int N, int *__r
         c4[0] = neg;
         c4[1] = neg;
         cell_level[c] = 0;
-        parent[c] = -1;
-        arrive[c] = 0;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *index = N + S;
@@ -300,7 +295,7 @@ __global__ void __launch_bounds__(256) keys_batch_kernel(
 void launch_batch_reset(nbx_batch *b, hipStream_t s)
 {
     reset_batch_kernel<<<grid_for(b->S, 256, 1024), 256, 0, s>>>(b->S, b->N, b->child, b->index, b->cell_level,
-                                                                 b->parent, b->arrive, b->st);
+                                                                 b->st);
 }
 
 void launch_batch_bbox(nbx_batch *b, hipStream_t s)

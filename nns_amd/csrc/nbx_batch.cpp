@@ -118,7 +118,7 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
               dalloc(&b->k2buf, N) && dalloc(&b->perm, N) && dalloc(&b->perm1, N) && dalloc(&b->vbuf, N) &&
               dalloc(&b->pos, N) && dalloc(&b->sorted, N) && dalloc(&b->hist, tiles * nbx::kRadix + nbx::kRadix) &&
               dalloc(&b->child, 8 * M) && dalloc(&b->index, 1) && dalloc(&b->count, M) && dalloc(&b->start, M) &&
-              dalloc(&b->cell_level, M) && dalloc(&b->parent, M) && dalloc(&b->arrive, M) &&
+              dalloc(&b->cell_level, M) &&
               dalloc(&b->mass64, M) && dalloc(&b->mom64, 3 * M) && dalloc(&b->nodes4, M) &&
               dalloc(&b->fnode, 8 * M) && dalloc(&b->fmeta, 8 * M) && dalloc(&b->froot4, S) &&
               dalloc(&b->frootm, S) && dalloc(&b->pblk, M) && dalloc(&b->bpar, N) && dalloc(&b->counts, N) && dalloc(&b->acc_scratch, 3 * N) && dalloc(&b->bodies4, N) &&
@@ -148,7 +148,7 @@ int nbx_batch_destroy(nbx_batch *b)
     drop_graph(b);
     void *bufs[] = {b->st, b->off, b->sys_of, b->theta, b->groups, b->ext, b->mm, b->keys, b->skeys, b->kbuf,
                     b->k2, b->k2buf, b->perm, b->perm1, b->vbuf, b->pos, b->sorted, b->hist, b->child, b->index,
-                    b->count, b->start, b->cell_level, b->parent, b->arrive, b->mass64, b->mom64, b->nodes4,
+                    b->count, b->start, b->cell_level, b->mass64, b->mom64, b->nodes4,
                     b->fnode, b->fmeta, b->froot4, b->frootm, b->pblk, b->bpar, b->counts, b->acc_scratch, b->samp_off, b->bodies4, b->gkey, b->gkey2, b->gorder};
     for (void *q : bufs)
         if (q) hipFree(q);

@@ -88,7 +88,7 @@ struct nbx_context {
     int32_t *perm = nullptr, *vbuf = nullptr, *pos = nullptr;
     uint32_t *hist = nullptr;
     int hist_tiles = 0;
-    int32_t *cell_level = nullptr, *parent = nullptr, *arrive = nullptr;
+    int32_t *cell_level = nullptr;
     double *mass64 = nullptr, *mom64 = nullptr;
     float4 *nodes4 = nullptr;
     float4 *fnode = nullptr;     // [8 * m_max] frame blocks of the force walk: child (x, y, z, m)
@@ -148,7 +148,7 @@ struct nbx_batch {
     int32_t *perm = nullptr, *perm1 = nullptr, *vbuf = nullptr, *pos = nullptr, *sorted = nullptr;
     uint32_t *hist = nullptr;
     int *child = nullptr, *index = nullptr, *count = nullptr, *start = nullptr;
-    int32_t *cell_level = nullptr, *parent = nullptr, *arrive = nullptr;
+    int32_t *cell_level = nullptr;
     double *mass64 = nullptr, *mom64 = nullptr;
     float4 *nodes4 = nullptr;
     float4 *fnode = nullptr;     // [8 * M] frame blocks
