@@ -154,6 +154,35 @@ def test_pipeline_matches_oracle(oracle, kind, n):
     check_forces(bh, t, oracle, float(np.float32(np.float32(params.softening) ** 2)))
 
 
+def _cluster_case(core, n, seed):
+    """`core` bodies in one tight cluster (sigma 1e-4 of an extent of about 2,
+    so the cell holding them sits near level 12 and every cell above it holds
+    all of them) and n - core spread bodies."""
+    r = np.random.default_rng(seed)
+    p = np.vstack([r.normal(0.3, 1e-4, (core, 3)), r.uniform(-1, 1, (n - core, 3))]).astype(np.float32)
+    z0 = np.zeros(n, np.float32)
+    m = r.uniform(0.5, 1.5, n).astype(np.float32) / n
+    return p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), z0, z0.copy(), z0.copy(), m
+
+
+@pytest.mark.parametrize("core,n", [(1024, 1100), (1025, 1100), (0, 1024), (0, 1025), (0, 2049), (2049, 2100)])
+def test_subtree_size_boundaries(oracle, core, n):
+    """Cells of exactly kSubtreeMax = 1024 bodies (one subtree) and of 1025 (a
+    top cell), alone or as a chain of top cells above a tight cluster, and body
+    counts just past a multiple of 1024 (the top-cell samples)."""
+    _torch()
+    print(f"case boundary core={core} n={n}", flush=True)
+    x, y, z, vx, vy, vz, m = _cluster_case(core, n, 11) if core else make_case("plummer", n)
+    params = nns_amd.default_params(theta=float(THETA))
+    bh = run_to_forces(x, y, z, vx, vy, vz, m, params)
+    t = oracle.KeyTree(x, y, z, m)
+    assert t.err == 0
+    assert np.array_equal(bh.ctx.copy_buffer(nns_amd.BUF_SORTED_KEYS, n, np.uint64), t.skeys)
+    assert np.array_equal(bh.host("sorted"), t.perm)
+    check_tree(bh, t)
+    check_forces(bh, t, oracle, float(np.float32(np.float32(params.softening) ** 2)))
+
+
 @pytest.mark.parametrize("kind,n", [("star_disk", 32768), ("coincident", 5000), ("plummer", 130)])
 def test_locked_build_matches_oracle(oracle, kind, n):
     """The lock-based subtree insertion (build_locked = 1) builds the same tree."""
