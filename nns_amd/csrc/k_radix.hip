@@ -3,14 +3,15 @@
 // Passes of kRadixBits = 8 bits.  Each pass is three launches over tiles of 2048
 // keys (4 waves x 8 items x 64 lanes; wave w owns the contiguous quarter w of the
 // tile, walked item by item so that lane order is element order):
-//   upsweep   per-tile digit histogram           reads 8 B/key (64-bit keys)
+//   upsweep   per-tile digit histogram (LDS atomics) reads 8 B/key (64-bit keys)
 //   scan      exclusive scan of each digit row       ~1 KiB per tile
 //   downsweep stable rank, the tile reordered by digit in LDS, then written out
 //             digit run by digit run (coalesced)  reads 12 B/key, writes 12 B/key
 // (32-bit keys: 4 and 8 B/key).
-// Ranks inside a wave come from a 64-lane match of the digit built from
-// kRadixBits ballots (no atomics, no intra-wave waits), so equal digits keep
-// their order.
+// In the downsweep, ranks inside a wave come from a 64-lane match of the digit
+// built from kRadixBits ballots (no atomics, no intra-wave waits), so equal
+// digits keep their order; the upsweep needs counts only and takes them with
+// LDS atomics.
 #include "nbx_internal.h"
 
 namespace nbx {
@@ -41,16 +42,19 @@ __global__ void __launch_bounds__(kSortThreads) radix_upsweep(
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int k = tid; k < (kSortThreads / 64) * kRadix; k += kSortThreads) (&cnt[0][0])[k] = 0u;
     __syncthreads();
+    // Counts only (no ranks): one LDS atomic per key into the wave's own row;
+    // lanes of a wave that share a digit are serialised by the LDS, which is
+    // never slower than building the 64-lane digit match out of ballots.
     const int64_t base = (int64_t)blockIdx.x * kSortTile + (int64_t)wave * (kSortItems * 64);
-#pragma unroll 4
+    uint32_t dig[kSortItems];
+#pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
         const int64_t e = base + i * 64 + lane;
-        const bool valid = e < n;
-        const uint32_t d = valid ? (uint32_t)((keys[e] >> shift) & (kRadix - 1)) : 0u;
-        const uint64_t live = __ballot(valid);
-        const uint64_t peers = match_digit(d, live);
-        if (valid && lane == __ffsll((unsigned long long)peers) - 1) cnt[wave][d] += (uint32_t)__popcll(peers);
+        dig[i] = e < n ? (uint32_t)((keys[e] >> shift) & (kRadix - 1)) : kRadix;  // kRadix: no key
     }
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i)
+        if (dig[i] < kRadix) atomicAdd(&cnt[wave][dig[i]], 1u);
     __syncthreads();
     for (int d = tid; d < kRadix; d += kSortThreads) {
         uint32_t s = 0;
