@@ -618,6 +618,77 @@ __device__ inline CellSum sum_cell(int c, int first, bool bucket, int n, int *ch
     return r;
 }
 
+// sum_cell with the loads of 4 slots issued before their first use: (1) the
+// cell children's sums, counts and first bodies (a slot without a cell loads
+// node 0, unused), (2) the body children at the cursor (the cell's first body
+// plus the sizes of the slots before them), (3) the sums in slot order, in
+// FP64, exactly as sum_cell adds them.  For the top summary, one workgroup
+// whose levels wait on these loads (the many-workgroup subtree summary keeps
+// sum_cell: the extra registers cost it more occupancy than the loads save).
+__device__ inline CellSum sum_cell_batched(int c, int first, int n, const int *child, const float4 *bodies4,
+                                           const float4 *sorted4, const int32_t *pos, const double *mass64,
+                                           const double *mom64, const int *count, const int *start)
+{
+    constexpr int B = 4;
+    int v[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) v[s] = child[8 * (int64_t)c + s];
+    CellSum r{0.0, 0.0, 0.0, 0.0, 0, 0x7fffffff};
+    int cur = first;
+#pragma unroll
+    for (int s0 = 0; s0 < 8; s0 += B) {
+        double cm[B], cx[B], cy[B], cz[B];
+        int kc[B], kf[B], bp[B];
+        float4 b[B];
+#pragma unroll
+        for (int t = 0; t < B; ++t) {
+            const int64_t q = v[s0 + t] >= n ? v[s0 + t] : 0;
+            cm[t] = mass64[q];
+            cx[t] = mom64[3 * q + 0];
+            cy[t] = mom64[3 * q + 1];
+            cz[t] = mom64[3 * q + 2];
+            kc[t] = count[q];
+            kf[t] = start[q];
+        }
+#pragma unroll
+        for (int t = 0; t < B; ++t) {
+            const int ch = v[s0 + t];
+            bp[t] = cur;
+            if (cur >= 0) cur += ch == -1 ? 0 : (ch < n ? 1 : kc[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < B; ++t) {
+            const int ch = v[s0 + t];
+            const bool body = ch >= 0 && ch < n;
+            const int p = bp[t];
+            const float4 *a = body && p >= 0 && p < n ? sorted4 + p : bodies4 + (body ? ch : 0);
+            b[t] = *a;
+        }
+#pragma unroll
+        for (int t = 0; t < B; ++t) {
+            const int ch = v[s0 + t];
+            if (ch == -1) continue;
+            if (ch < n) {
+                const double mb = (double)b[t].w;
+                r.ms = __dadd_rn(r.ms, mb);
+                r.mx = __dadd_rn(r.mx, __dmul_rn(mb, (double)b[t].x));
+                r.my = __dadd_rn(r.my, __dmul_rn(mb, (double)b[t].y));
+                r.mz = __dadd_rn(r.mz, __dmul_rn(mb, (double)b[t].z));
+                r.cnt += 1;
+                r.first = min(r.first, bp[t] >= 0 ? bp[t] : pos[ch]);
+            } else {
+                r.ms = __dadd_rn(r.ms, cm[t]);
+                r.mx = __dadd_rn(r.mx, cx[t]);
+                r.my = __dadd_rn(r.my, cy[t]);
+                r.mz = __dadd_rn(r.mz, cz[t]);
+                r.cnt += kc[t];
+                r.first = min(r.first, kf[t]);
+            }
+        }
+    }
+    return r;
+}
+
 struct SumOut {
     int n;
     int *child;
@@ -633,13 +704,18 @@ struct SumOut {
 
 // lev: the cell's level when the caller has it (else -1); lslots: an LDS copy of
 // its slots (else nullptr).
+// BATCHED: the cell is not a bucket and its slots are read from child[]
+// (the top cells and the roots), summed with sum_cell_batched.
+template <bool BATCHED = false>
 __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool single_root, int root_first = 0,
                                  int lev = -1, const int *lslots = nullptr)
 {
-    const bool bucket = !single_root && (lev >= 0 ? lev : o.cell_level[c]) == kKeyLevels;
+    const bool bucket = !BATCHED && !single_root && (lev >= 0 ? lev : o.cell_level[c]) == kKeyLevels;
     const int first = !o.cursor ? -1 : single_root ? root_first : o.start[c];
-    const CellSum r = sum_cell(c, first, bucket, o.n, o.child, lslots, o.bodies4, o.nodes4, o.pos, o.mass64,
-                               o.mom64, o.count, o.start);
+    const CellSum r = BATCHED ? sum_cell_batched(c, first, o.n, o.child, o.bodies4, o.nodes4, o.pos, o.mass64,
+                                                 o.mom64, o.count, o.start)
+                              : sum_cell(c, first, bucket, o.n, o.child, lslots, o.bodies4, o.nodes4, o.pos,
+                                         o.mass64, o.mom64, o.count, o.start);
     if (r.cnt > 0 && !(r.ms > 0.0)) atomicOr(&st->err, kErrNonFinite);
     const double gx = r.mx / r.ms, gy = r.my / r.ms, gz = r.mz / r.ms;
     const float fx = (float)gx, fy = (float)gy, fz = (float)gz, fm = (float)r.ms;
@@ -708,10 +784,10 @@ __global__ void __launch_bounds__(1024) top_summary_kernel(SumOut o, Segs g, int
     for (int L = kKeyLevels; L >= 1; --L) {
         const int cnt = s_cnt[L];
         if (cnt == 0) continue;  // uniform: no barrier for an empty level
-        for (int i = threadIdx.x; i < cnt; i += blockDim.x) summarise(o, top_list[L * nsamples + i], st, false);
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) summarise<true>(o, top_list[L * nsamples + i], st, false);
         __syncthreads();
     }
-    for (int s = threadIdx.x; s < g.nseg; s += blockDim.x) summarise(o, seg_root(g, s), st, !g.batch);
+    for (int s = threadIdx.x; s < g.nseg; s += blockDim.x) summarise<true>(o, seg_root(g, s), st, !g.batch);
 }
 
 // ------------------------------------------------------------------ layout
