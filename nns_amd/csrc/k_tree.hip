@@ -868,6 +868,30 @@ __device__ inline int2 chain_bottom(int c, int n, const int *__restrict__ child,
     }
 }
 
+// chain_bottom of a cell child whose slots (a, b) and level are already loaded:
+// the loads of a chain are made only when the child has a single cell child.
+template <bool BATCH>
+__device__ inline int2 chain_bottom_from(int c, int4 a, int4 b, int lev, int n, const int *__restrict__ child,
+                                         const int32_t *__restrict__ cell_level, double span, float theta,
+                                         float eps2)
+{
+    for (;;) {
+        const int v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        int cnt = 0, last = -1;
+        for (int s = 0; s < 8; ++s)
+            if (v[s] != -1) { ++cnt; last = v[s]; }
+        if (cnt == 1 && last >= n) {
+            c = last;
+            const int4 *p = reinterpret_cast<const int4 *>(child + 8 * (int64_t)c);
+            a = p[0];
+            b = p[1];
+            lev = cell_level[c];
+            continue;
+        }
+        return make_int2((block_of<BATCH>(c, n) << 4) | cnt, __float_as_int(crit_of(span, theta, eps2, lev)));
+    }
+}
+
 // Ensemble: the system of a cell from its first sorted body.
 struct SysInfo {
     const int32_t *start, *perm, *sys_of;
@@ -896,13 +920,23 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
         const int s = (int)(k & 7);
         const int node = BATCH ? n + (int)item : (item == 0 ? 0 : n + (int)item - 1);
         const int ch = k < total ? child[8 * (int64_t)node + s] : -1;
+        // Everything a cell child needs (its slots, level, count and centre) and
+        // the node's first sorted body are loaded now, before the scan below: a
+        // slot that holds no cell reads node 0, unused.
+        const int64_t cq = ch >= n ? ch : 0;
+        const int4 *cs4 = reinterpret_cast<const int4 *>(child + 8 * cq);
+        const int4 ca = cs4[0], cb = cs4[1];
+        const int clev = cell_level[cq];
+        const int ccnt = count[cq];
+        const float4 cnode = nodes4[cq];
+        const int nstart = k >= total || (!BATCH && item == 0) ? 0 : start[node];  // the single root is node 0
         const uint64_t occ = __ballot(ch != -1);
         const unsigned seg = (unsigned)(occ >> (lane & ~7)) & 0xffu;
         const int rank = __popc(seg & ((1u << s) - 1u));
         // Sorted position of a body child: the node's first sorted body plus the
         // bodies of the slots before it (children in slot order are consecutive
         // ranges of the key order).  8-lane exclusive sum of the children's sizes.
-        const int w = ch == -1 ? 0 : ch < n ? 1 : count[ch];
+        const int w = ch == -1 ? 0 : ch < n ? 1 : ccnt;
         int off = w;
 #pragma unroll
         for (int o = 1; o < 8; o <<= 1) {
@@ -929,15 +963,14 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
         if (ch == -1) continue;
         const int64_t slot = 8 * item + rank;
         if (ch < n) {
-            const int first = (!BATCH && item == 0) ? 0 : start[node];  // the single root is node 0
-            const int q = first + off;
+            const int q = nstart + off;
             const int ps = q >= 0 && q < n ? q : pos[ch];
             fnode[slot] = nodes4[ps];
             fmeta[slot] = make_int2(ps << 4, body_crit);
             bpar[ps] = (int)item;
         } else {
-            fnode[slot] = nodes4[ch];
-            fmeta[slot] = chain_bottom<BATCH>(ch, n, child, cell_level, false, span, th, eps2);
+            fnode[slot] = cnode;
+            fmeta[slot] = chain_bottom_from<BATCH>(ch, ca, cb, clev, n, child, cell_level, span, th, eps2);
             pblk[block_of<BATCH>(ch, n)] = (int)item;
         }
     }
