@@ -572,11 +572,19 @@ struct CellSum {
 // bodies, so with the cell's first sorted position known (first >= 0) a body
 // child is read from the sorted copy at a running cursor; otherwise (first < 0)
 // from its own index, with its sorted position from pos[].
+// The slots go in batches of B (a divisor of 8): per batch (1) the cell
+// children's sums, counts and first bodies, every load issued before the
+// first use (a slot without a cell loads node 0, unused), (2) the body
+// children at the cursor (the cell's first body plus the sizes of the slots
+// before them), (3) the sums in slot order, in FP64.  B = 2 for the subtrees
+// (more would cost occupancy: 4 needs 90 registers against 59), 4 for the
+// top cells (one workgroup).
+template <int B>
 __device__ inline CellSum sum_cell(int c, int first, bool bucket, int n, int *child, const int *lslots,
                                    const float4 *bodies4, const float4 *sorted4, const int32_t *pos,
                                    const double *mass64, const double *mom64, const int *count, const int *start)
 {
-    int v[8];  // the slots, from the LDS copy when there is one
+    int v[8];
     const int *src = lslots ? lslots : child + 8 * (int64_t)c;
 #pragma unroll
     for (int s = 0; s < 8; ++s) v[s] = src[s];
@@ -588,51 +596,6 @@ __device__ inline CellSum sum_cell(int c, int first, bool bucket, int n, int *ch
         }
         for (int s = 0; s < 8; ++s) child[8 * (int64_t)c + s] = v[s];
     }
-    CellSum r{0.0, 0.0, 0.0, 0.0, 0, 0x7fffffff};
-    int cur = first;
-#pragma unroll
-    for (int s = 0; s < 8; ++s) {
-        const int ch = v[s];
-        if (ch == -1) continue;
-        if (ch < n) {
-            const float4 b = cur >= 0 && cur < n ? sorted4[cur] : bodies4[ch];
-            const double mb = (double)b.w;
-            r.ms = __dadd_rn(r.ms, mb);
-            r.mx = __dadd_rn(r.mx, __dmul_rn(mb, (double)b.x));
-            r.my = __dadd_rn(r.my, __dmul_rn(mb, (double)b.y));
-            r.mz = __dadd_rn(r.mz, __dmul_rn(mb, (double)b.z));
-            r.cnt += 1;
-            r.first = min(r.first, cur >= 0 ? cur : pos[ch]);
-            if (cur >= 0) cur += 1;
-        } else {
-            r.ms = __dadd_rn(r.ms, mass64[ch]);
-            r.mx = __dadd_rn(r.mx, mom64[3 * (int64_t)ch + 0]);
-            r.my = __dadd_rn(r.my, mom64[3 * (int64_t)ch + 1]);
-            r.mz = __dadd_rn(r.mz, mom64[3 * (int64_t)ch + 2]);
-            const int k = count[ch];
-            r.cnt += k;
-            r.first = min(r.first, start[ch]);
-            if (cur >= 0) cur += k;
-        }
-    }
-    return r;
-}
-
-// sum_cell with the loads of 4 slots issued before their first use: (1) the
-// cell children's sums, counts and first bodies (a slot without a cell loads
-// node 0, unused), (2) the body children at the cursor (the cell's first body
-// plus the sizes of the slots before them), (3) the sums in slot order, in
-// FP64, exactly as sum_cell adds them.  For the top summary, one workgroup
-// whose levels wait on these loads (the many-workgroup subtree summary keeps
-// sum_cell: the extra registers cost it more occupancy than the loads save).
-__device__ inline CellSum sum_cell_batched(int c, int first, int n, const int *child, const float4 *bodies4,
-                                           const float4 *sorted4, const int32_t *pos, const double *mass64,
-                                           const double *mom64, const int *count, const int *start)
-{
-    constexpr int B = 4;
-    int v[8];
-#pragma unroll
-    for (int s = 0; s < 8; ++s) v[s] = child[8 * (int64_t)c + s];
     CellSum r{0.0, 0.0, 0.0, 0.0, 0, 0x7fffffff};
     int cur = first;
 #pragma unroll
@@ -704,18 +667,15 @@ struct SumOut {
 
 // lev: the cell's level when the caller has it (else -1); lslots: an LDS copy of
 // its slots (else nullptr).
-// BATCHED: the cell is not a bucket and its slots are read from child[]
-// (the top cells and the roots), summed with sum_cell_batched.
-template <bool BATCHED = false>
+// B: slots whose loads go together (sum_cell).
+template <int B>
 __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool single_root, int root_first = 0,
                                  int lev = -1, const int *lslots = nullptr)
 {
-    const bool bucket = !BATCHED && !single_root && (lev >= 0 ? lev : o.cell_level[c]) == kKeyLevels;
+    const bool bucket = !single_root && (lev >= 0 ? lev : o.cell_level[c]) == kKeyLevels;
     const int first = !o.cursor ? -1 : single_root ? root_first : o.start[c];
-    const CellSum r = BATCHED ? sum_cell_batched(c, first, o.n, o.child, o.bodies4, o.nodes4, o.pos, o.mass64,
-                                                 o.mom64, o.count, o.start)
-                              : sum_cell(c, first, bucket, o.n, o.child, lslots, o.bodies4, o.nodes4, o.pos,
-                                         o.mass64, o.mom64, o.count, o.start);
+    const CellSum r = sum_cell<B>(c, first, bucket, o.n, o.child, lslots, o.bodies4, o.nodes4, o.pos, o.mass64,
+                                  o.mom64, o.count, o.start);
     if (r.cnt > 0 && !(r.ms > 0.0)) atomicOr(&st->err, kErrNonFinite);
     const double gx = r.mx / r.ms, gy = r.my / r.ms, gz = r.mz / r.ms;
     const float fx = (float)gx, fy = (float)gy, fz = (float)gz, fm = (float)r.ms;
@@ -740,6 +700,7 @@ __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool s
 // cells first copies its cells' levels and slots into LDS (one coalesced read),
 // so a level costs the reads of the children's sums and the stores only.
 constexpr int kSumLdsCells = 512;
+constexpr int kSubSumBatch = 2;
 __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const int4 *__restrict__ subcells,
                                                               DeviceState *st)
 {
@@ -766,7 +727,7 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
         for (int L = sc.w; L >= L0; --L) {
             for (int k = tid; k < ks; k += blockDim.x) {
                 const int lev = lds ? s_lev[k] : o.cell_level[base + k];
-                if (lev == L) summarise(o, base + k, st, false, 0, lev, lds ? s_slot + 8 * k : nullptr);
+                if (lev == L) summarise<kSubSumBatch>(o, base + k, st, false, 0, lev, lds ? s_slot + 8 * k : nullptr);
             }
             __syncthreads();
         }
@@ -784,10 +745,10 @@ __global__ void __launch_bounds__(1024) top_summary_kernel(SumOut o, Segs g, int
     for (int L = kKeyLevels; L >= 1; --L) {
         const int cnt = s_cnt[L];
         if (cnt == 0) continue;  // uniform: no barrier for an empty level
-        for (int i = threadIdx.x; i < cnt; i += blockDim.x) summarise<true>(o, top_list[L * nsamples + i], st, false);
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) summarise<4>(o, top_list[L * nsamples + i], st, false);
         __syncthreads();
     }
-    for (int s = threadIdx.x; s < g.nseg; s += blockDim.x) summarise<true>(o, seg_root(g, s), st, !g.batch);
+    for (int s = threadIdx.x; s < g.nseg; s += blockDim.x) summarise<4>(o, seg_root(g, s), st, !g.batch);
 }
 
 // ------------------------------------------------------------------ layout
