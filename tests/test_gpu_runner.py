@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 RUNNER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "nns_amd", "nbx_runner")
 
 
-@pytest.mark.parametrize("extra", [[], ["--locked-build", "--model", "plummer"]])
+@pytest.mark.parametrize("extra", [[], ["--locked-build", "--model", "plummer"], ["--copy", "--model", "star_disk"]])
 def test_runner_reference_options_and_time_lines(extra):
     assert os.access(RUNNER, os.X_OK), "nns_amd/nbx_runner is not built (python -m nns_amd.build --runner)"
     cmd = [RUNNER, "-n", "8192", "-i", "3", "-b", "128", "-g", "512", "-R", "5", "-r", "-v", *extra]
@@ -28,4 +28,6 @@ def test_runner_reference_options_and_time_lines(extra):
                   "compute forces", "update bodies", "TOTAL"):
         assert f"[TIME ] Time to {stage}:" in r.stdout
     assert "[INFO ] Number of particles: 8192" in r.stdout
+    copy = re.search(r"Time to copy from device to host: ([0-9.]+) ms", r.stdout)
+    assert copy and (float(copy.group(1)) > 0) == ("--copy" in extra)  # the reference's per-step D2H copy
 
