@@ -16,7 +16,9 @@ l/D < theta plus leaf bodies) counted on the GPU outside the timed region.
 MI355X FP32 vector peak; `cpu_baseline` times the reference's own OpenMP tree
 code (oracle/_ref/ref_harness) on this box's host cores on a bounded sample,
 with the bodies walked in key order (the reference's `hashed` path, the faster
-of its two; index order is reported beside it).
+of its two; index order is reported beside it).  `accuracy` holds the error
+against an FP64 direct sum on a seeded sample, at theta 0.5 and, with the
+force time, at theta 0.25.
 
 Beside the headline the line carries the other forms, each on its own
 workload:
@@ -410,6 +412,31 @@ def main():
         rg = np.linalg.norm(agpu - ab, axis=1) / np.linalg.norm(ab, axis=1)
         accuracy["fp64_tree_rms_rel_err_vs_fp64_direct"] = float(np.sqrt((rb ** 2).mean()))
         accuracy["rms_rel_err_vs_fp64_tree"] = float(np.sqrt((rg ** 2).mean()))
+        # the same tree and sample at theta = 0.25: what an RMS error under 1e-3
+        # costs with the reference's opening test (the force stage alone, timed
+        # after one untimed call that also lays the frames out for the new theta)
+        bh.ctx.set_params(nns_amd.default_params(theta=0.25, kahan=params.kahan))
+
+        def forces():
+            kw.computeForces(a2["x"], a2["y"], a2["z"], a2["vx"], a2["vy"], a2["vz"], a2["ax"], a2["ay"],
+                             a2["az"], a2["mass"], a2["sorted"], a2["child"], a2["minX"], a2["maxX"], n,
+                             params.gravity)
+        forces()
+        f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        f0.record(stream)
+        for _ in range(3):
+            forces()
+        f1.record(stream)
+        torch.cuda.synchronize()
+        bh.ctx.check()
+        f25 = f0.elapsed_time(f1) / 3
+        a25 = np.stack([bh.host("ax")[tgt], bh.host("ay")[tgt], bh.host("az")[tgt]], 1).astype(np.float64)
+        r25 = np.linalg.norm(a25 - ad, axis=1) / np.linalg.norm(ad, axis=1)
+        i25 = int(bh.count_interactions().sum().item())
+        accuracy["theta_0.25"] = {"rms_rel_err_vs_fp64_direct": float(np.sqrt((r25 ** 2).mean())),
+                                  "max_rel_err": float(r25.max()), "force_ms": round(f25, 4),
+                                  "interactions": i25, "g_interactions_per_s": round(i25 / f25 / 1e6, 2)}
+        bh.ctx.set_params(params)
 
     cpu = None
     if args.cpu_sample > 0:
