@@ -53,6 +53,30 @@ def test_vjp_matches_oracle(kind, n, theta):
     assert rel_err(gm, ogm) <= 2e-5, rel_err(gm, ogm)
 
 
+@pytest.mark.parametrize("kind,n", [("plummer", 4096), ("coincident", 3000)])
+def test_vjp_unsoftened_matches_oracle(kind, n):
+    """eps = 0: the walk compiled without softening, where a body at the same
+    point as the lane's own is left out (|d|^2 = 0), as in the forward walk."""
+    torch = _torch()
+    print(f"case vjp unsoftened {kind} n={n}", flush=True)
+    x, y, z, vx, vy, vz, m = make_case(kind, n, seed=3)
+    p = nns_amd.default_params(theta=0.5, softening=0.0)
+    m_nodes = 24 * n + 12000 if kind == "coincident" else None
+    bh = nns_amd.BarnesHut(x, y, z, vx, vy, vz, m, params=p, m=m_nodes)
+    bh.rebuild()
+    u = np.random.default_rng(8).normal(size=(n, 3)).astype(np.float32)
+    ut = [torch.as_tensor(u[:, k].copy(), device=bh.dev) for k in range(3)]
+    gx, gy, gz, gm = [t.cpu().numpy() for t in bh.vjp(*ut)]
+    bh.ctx.check()
+    cap = 23 * n + 12000 if kind == "coincident" else 0
+    t = O.KeyTree(x, y, z, m, cap=cap)
+    ogx, ogm = t.vjp(0.5, 0.0, u.astype(np.float64), mac_mode=1)
+    g = np.stack([gx, gy, gz], 1)
+    assert np.isfinite(g).all() and np.isfinite(gm).all()
+    assert rel_err(g, ogx) <= 2e-5, rel_err(g, ogx)
+    assert rel_err(gm, ogm) <= 2e-5, rel_err(gm, ogm)
+
+
 def _direct_accel_t(torch, x, y, z, m, eps2):
     dx = x[None, :] - x[:, None]
     dy = y[None, :] - y[:, None]
