@@ -89,7 +89,8 @@ void nbx_default_params(nbx_params *p);
 
 /* Creates a context for up to n_max bodies and m_max nodes (the reference uses
  * m = 2n + 12000, src/BarnesHut.cu:18).  device: HIP device ordinal the context
- * allocates on and launches on. */
+ * allocates on and launches on.  n_max is at most 48,805,888 (NBX_ERR_INVALID
+ * above): the tree's top table holds 2^20 entries, 22 per 1024 bodies. */
 int nbx_create(const nbx_params *p, int n_max, int m_max, int device, nbx_context **out);
 int nbx_destroy(nbx_context *ctx);
 int nbx_set_params(nbx_context *ctx, const nbx_params *p);
@@ -216,7 +217,8 @@ int nbx_force_vjp(nbx_context *ctx, const float *ux, const float *uy, const floa
 /* nsys independent systems in one set of concatenated arrays: system s owns bodies
  * [off_s, off_s + sys_n[s]) with off_s = sys_n[0] + ... + sys_n[s-1].  sys_n and sys_theta
  * (per-system opening angle; NULL = p->theta) are host arrays.  Every stage is one launch
- * for the whole ensemble; the tree of system s hangs off its own root cell. */
+ * for the whole ensemble; the tree of system s hangs off its own root cell.  At most
+ * 65535 systems and 47,662 samples: the sum over systems of ceil(sys_n[s] / 1024). */
 typedef struct nbx_batch nbx_batch;
 int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const float *sys_theta, int device,
                      nbx_batch **out);

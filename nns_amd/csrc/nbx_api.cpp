@@ -103,6 +103,7 @@ int nbx_create(const nbx_params *p, int n_max, int m_max, int device, nbx_contex
     *out = nullptr;
     if (!params_ok(p)) return fail(nullptr, NBX_ERR_INVALID, "invalid parameters");
     if (n_max < 1 || m_max < n_max + 1) return fail(nullptr, NBX_ERR_INVALID, "need 1 <= n_max < m_max");
+    if (n_max > nbx::kMaxBodies) return fail(nullptr, NBX_ERR_INVALID, "n_max above the maximum of 48,805,888 bodies");
     nbx_context *c = new (std::nothrow) nbx_context();
     if (!c) return fail(nullptr, NBX_ERR_INVALID, "out of host memory");
     c->p = *p;

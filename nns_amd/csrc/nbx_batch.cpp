@@ -80,11 +80,14 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
     *out = nullptr;
     if (!p || nsys < 1 || !sys_n) return bfail(nullptr, NBX_ERR_INVALID, "need params, nsys >= 1 and sys_n");
     if (nsys > 65535) return bfail(nullptr, NBX_ERR_INVALID, "at most 65535 systems");
-    int64_t total = 0;
+    int64_t total = 0, samples = 0;
     for (int s = 0; s < nsys; ++s) {
         if (sys_n[s] < 1) return bfail(nullptr, NBX_ERR_INVALID, "every system needs >= 1 body");
         total += sys_n[s];
+        samples += (sys_n[s] + nbx::kSubtreeMax - 1) / nbx::kSubtreeMax;
     }
+    if (samples > nbx::kMaxSamples)
+        return bfail(nullptr, NBX_ERR_INVALID, "ensemble above the maximum of 47,662 samples (sum of ceil(n_s / 1024))");
     const int64_t m64 = 2 * total + nsys + 12000;
     if (m64 * 8 >= (int64_t)1 << 31) return bfail(nullptr, NBX_ERR_INVALID, "ensemble too large");
     if (sys_theta)

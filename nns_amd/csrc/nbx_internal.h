@@ -39,6 +39,11 @@ constexpr int kErrNonFinite = 8;
 // cell with 2..kSubtreeMax bodies is a subtree built by one workgroup.
 constexpr int kSubtreeMax = 1024;
 constexpr int kLevels = kKeyLevels + 1;  // levels 0..21 per sample in the top table
+// The top table (kLevels entries per sample of kSubtreeMax bodies) is scanned by
+// a two-level scan of at most 2^20 entries: the largest system, or ensemble
+// (each system's samples rounded up), is 47,662 samples = 48,805,888 bodies.
+constexpr int kMaxSamples = (1 << 20) / kLevels;
+constexpr int kMaxBodies = kMaxSamples * kSubtreeMax;
 
 struct DeviceState {
     double ext[4];      // xMin, yMin, zMin, maxSpan

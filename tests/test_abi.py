@@ -48,6 +48,22 @@ def test_create_validates_before_device():
         bad = nns_amd.default_params(block_size=bs)
         assert L.nbx_create(C.byref(bad), 100, 1000, 0, C.byref(h)) == -1
         assert b"invalid parameters" in L.nbx_last_error(None)
+    # the largest system: 47,662 samples of 1024 bodies fill the 2^20-entry top table
+    assert L.nbx_create(C.byref(p), 48805889, 2 * 48805889 + 12000, 0, C.byref(h)) == -1
+    assert b"maximum of 48,805,888" in L.nbx_last_error(None)
+    assert not h.value
+
+
+def test_batch_create_validates_before_device():
+    L = nns_amd.lib()
+    p = nns_amd.default_params()
+    h = C.c_void_p()
+    ones = (C.c_int * 47663)(*([1] * 47663))  # one sample per system: one over the maximum
+    assert L.nbx_batch_create(C.byref(p), 47663, ones, None, 0, C.byref(h)) == -1
+    assert b"47,662 samples" in L.nbx_batch_last_error(None)
+    sizes = (C.c_int * 2)(5, 0)
+    assert L.nbx_batch_create(C.byref(p), 2, sizes, None, 0, C.byref(h)) == -1
+    assert b">= 1 body" in L.nbx_batch_last_error(None)
     assert not h.value
 
 

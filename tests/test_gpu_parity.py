@@ -224,6 +224,60 @@ def test_full_size_binary_disk(oracle):
     assert np.array_equal(bh.count_interactions().cpu().numpy()[tg], cnt_orc)
 
 
+def test_maximum_size(oracle):
+    """The largest system a context takes, N = 48,805,888 (47,662 samples of
+    1024 bodies fill the tree's 2^20-entry top table): a Gaussian sphere at
+    theta 0.5.  Sorted keys and order bit for bit, through the properties that
+    fix a stable sort (keys in order and equal to the oracle's keys of the
+    listed bodies, a permutation, equal keys in index order); the FP64 root mass
+    and centre of mass against FP64 sums; accelerations of 256 seeded bodies
+    against an FP64 direct sum, to the Barnes-Hut error at theta 0.5 (the
+    oracle's walk of a 200K Gaussian sphere: RMS 1.5e-3, max 7.5e-3)."""
+    torch = _torch()
+    n = 48805888
+    print(f"case gaussian n={n}", flush=True)
+    r = np.random.default_rng(11)
+    p = r.normal(size=(n, 3)).astype(np.float32)
+    x, y, z = (p[:, k].copy() for k in range(3))
+    del p
+    m = np.full(n, 1.0 / n, np.float32)
+    z0 = np.zeros(n, np.float32)
+    params = nns_amd.default_params(theta=0.5)
+    bh = run_to_forces(x, y, z, z0, z0, z0, m, params)
+    keys = oracle.keys_morton(x, y, z)
+    sk = bh.ctx.copy_buffer(nns_amd.BUF_SORTED_KEYS, n, np.uint64)
+    srt = bh.host("sorted").astype(np.int64)
+    assert np.array_equal(sk, keys[srt])
+    seen = np.zeros(n, bool)
+    seen[srt] = True
+    assert seen.all()
+    up, eq = sk[1:] > sk[:-1], sk[1:] == sk[:-1]
+    assert (up | (eq & (srt[1:] > srt[:-1]))).all()
+    del keys, sk, srt, seen, up, eq
+    m64 = m.astype(np.float64)
+    M = m64.sum()
+    root = bh.ctx.copy_buffer(nns_amd.BUF_ROOT, 4, np.float64)
+    assert abs(root[0] - M) <= 1e-12 * M
+    for k, v in enumerate((x, y, z)):
+        assert abs(root[1 + k] - np.dot(m64, v.astype(np.float64)) / M) <= 1e-10
+    tg = np.sort(r.choice(n, 256, replace=False))
+    dev = bh.dev
+    P = torch.stack([torch.as_tensor(v, device=dev).double() for v in (x, y, z)], 1)
+    W = torch.as_tensor(m64, device=dev)
+    T = P[torch.as_tensor(tg, device=dev)]
+    eps2 = float(np.float32(np.float32(params.softening) ** 2))
+    acc = torch.zeros(256, 3, dtype=torch.float64, device=dev)
+    for lo in range(0, n, 1 << 18):
+        d = P[lo:lo + (1 << 18)][None, :, :] - T[:, None, :]
+        w = W[lo:lo + (1 << 18)][None, :] * ((d * d).sum(-1) + eps2).pow(-1.5)
+        acc += (w[..., None] * d).sum(1)
+    ref = params.gravity * acc.cpu().numpy()
+    ag = np.stack([bh.host("ax")[tg], bh.host("ay")[tg], bh.host("az")[tg]], 1).astype(np.float64)
+    rel = np.linalg.norm(ag - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    print(f"direct-sum error rms {np.sqrt((rel ** 2).mean()):.3e} max {rel.max():.3e}", flush=True)
+    assert np.sqrt((rel ** 2).mean()) <= 3e-3 and rel.max() <= 2e-2
+
+
 def test_unsoftened_and_no_kahan(oracle):
     _torch()
     x, y, z, vx, vy, vz, m = make_case("coincident", 3000, seed=5)
