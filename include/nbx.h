@@ -90,7 +90,9 @@ void nbx_default_params(nbx_params *p);
 /* Creates a context for up to n_max bodies and m_max nodes (the reference uses
  * m = 2n + 12000, src/BarnesHut.cu:18).  device: HIP device ordinal the context
  * allocates on and launches on.  n_max is at most 48,805,888 (NBX_ERR_INVALID
- * above): the tree's top table holds 2^20 entries, 22 per 1024 bodies. */
+ * above): the tree's top table holds 2^20 entries, 22 per 1024 bodies.  m_max is
+ * at most 2^27 = 134,217,728: the force walk's node descriptors hold a node's
+ * block index times 16 in 32 bits. */
 int nbx_create(const nbx_params *p, int n_max, int m_max, int device, nbx_context **out);
 int nbx_destroy(nbx_context *ctx);
 int nbx_set_params(nbx_context *ctx, const nbx_params *p);

@@ -104,6 +104,7 @@ int nbx_create(const nbx_params *p, int n_max, int m_max, int device, nbx_contex
     if (!params_ok(p)) return fail(nullptr, NBX_ERR_INVALID, "invalid parameters");
     if (n_max < 1 || m_max < n_max + 1) return fail(nullptr, NBX_ERR_INVALID, "need 1 <= n_max < m_max");
     if (n_max > nbx::kMaxBodies) return fail(nullptr, NBX_ERR_INVALID, "n_max above the maximum of 48,805,888 bodies");
+    if (m_max > nbx::kMaxNodes) return fail(nullptr, NBX_ERR_INVALID, "m_max above the maximum of 134,217,728 nodes");
     nbx_context *c = new (std::nothrow) nbx_context();
     if (!c) return fail(nullptr, NBX_ERR_INVALID, "out of host memory");
     c->p = *p;

@@ -89,7 +89,7 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
     if (samples > nbx::kMaxSamples)
         return bfail(nullptr, NBX_ERR_INVALID, "ensemble above the maximum of 47,662 samples (sum of ceil(n_s / 1024))");
     const int64_t m64 = 2 * total + nsys + 12000;
-    if (m64 * 8 >= (int64_t)1 << 31) return bfail(nullptr, NBX_ERR_INVALID, "ensemble too large");
+    if (m64 > nbx::kMaxNodes) return bfail(nullptr, NBX_ERR_INVALID, "ensemble above the maximum of 134,217,728 nodes");
     if (sys_theta)
         for (int s = 0; s < nsys; ++s)
             if (!(sys_theta[s] >= 0.f)) return bfail(nullptr, NBX_ERR_INVALID, "invalid theta");

@@ -44,6 +44,9 @@ constexpr int kLevels = kKeyLevels + 1;  // levels 0..21 per sample in the top t
 // (each system's samples rounded up), is 47,662 samples = 48,805,888 bodies.
 constexpr int kMaxSamples = (1 << 20) / kLevels;
 constexpr int kMaxBodies = kMaxSamples * kSubtreeMax;
+// A frame descriptor's tag holds a block index (or a sorted position) times 16
+// in an int: nodes stay below 2^27.
+constexpr int kMaxNodes = 1 << 27;
 
 struct DeviceState {
     double ext[4];      // xMin, yMin, zMin, maxSpan

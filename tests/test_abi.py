@@ -51,6 +51,8 @@ def test_create_validates_before_device():
     # the largest system: 47,662 samples of 1024 bodies fill the 2^20-entry top table
     assert L.nbx_create(C.byref(p), 48805889, 2 * 48805889 + 12000, 0, C.byref(h)) == -1
     assert b"maximum of 48,805,888" in L.nbx_last_error(None)
+    assert L.nbx_create(C.byref(p), 1000, (1 << 27) + 1, 0, C.byref(h)) == -1
+    assert b"maximum of 134,217,728 nodes" in L.nbx_last_error(None)
     assert not h.value
 
 
