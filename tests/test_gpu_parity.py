@@ -278,6 +278,35 @@ def test_maximum_size(oracle):
     assert np.sqrt((rel ** 2).mean()) <= 3e-3 and rel.max() <= 2e-2
 
 
+def test_step_captured_in_graph_matches_eager():
+    """nbx_step makes no host synchronisation, copy, memset or allocation when
+    not timed, so a caller can capture it on its stream into a graph.  Each of
+    three replays of one captured step, with eager work and synchronisations
+    between them, matches the eager step of a twin system bit for bit."""
+    torch = _torch()
+    n = 30000
+    print(f"case graph capture plummer n={n}", flush=True)
+    x, y, z, vx, vy, vz, m = nns_amd.initial_conditions("plummer", n, 4)
+    eager = nns_amd.BarnesHut(x, y, z, vx, vy, vz, m)
+    graph = nns_amd.BarnesHut(x, y, z, vx, vy, vz, m)
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g, stream=side):
+        graph.update(stream=side.cuda_stream)
+    torch.cuda.synchronize()
+    for k in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+        eager.update()
+        torch.cuda.synchronize()
+        _ = torch.ones(1000, device="cuda").sum().item()  # eager work between replays
+        for key in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"):
+            assert np.array_equal(eager.host(key), graph.host(key)), (k, key)
+    eager.ctx.check()
+    graph.ctx.check()
+
+
 def test_unsoftened_and_no_kahan(oracle):
     _torch()
     x, y, z, vx, vy, vz, m = make_case("coincident", 3000, seed=5)
