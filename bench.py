@@ -13,7 +13,8 @@ Prints one JSON line.  `value` is G interactions/s of the force stage, where
 interactions are the per-particle interaction lists (cells accepted by
 l/D < theta plus leaf bodies) counted on the GPU outside the timed region.
 `roofline` prices the force kernel at 20 flop per interaction against the
-MI355X FP32 vector peak; `cpu_baseline` times the reference's own OpenMP tree
+MI355X FP32 vector peak, and carries the measured issue rates of scalar
+v_fma_f32 and of v_rsq_f32 on this device (nbx_probe_valu); `cpu_baseline` times the reference's own OpenMP tree
 code (oracle/_ref/ref_harness) on this box's host cores on a bounded sample,
 with the bodies walked in key order (the reference's `hashed` path, the faster
 of its two; index order is reported beside it).  `accuracy` holds the error
@@ -292,6 +293,18 @@ def run_adjoint(args):
                          "frac_within_1e-2": float((rel < 1e-2).mean()), "max_rel_err": float(rel.max())}}
 
 
+def valu_probe():
+    """Measured FP32 VALU rates of this device (best of 3 launches of 2M lanes x
+    32,768 instructions): scalar v_fma_f32 in TFLOP/s (packed FP32 is disabled
+    for the library's device code), and v_rsq_f32 as a fraction of the FMA
+    issue rate."""
+    import nns_amd
+    fma = max(nns_amd.probe_valu("fma")[0] for _ in range(3))
+    rsq = max(nns_amd.probe_valu("rsq")[0] for _ in range(3))
+    return {"measured_scalar_fp32_fma_tflops": round(2 * fma / 1e12, 2),
+            "measured_rsq_over_fma_issue": round(rsq / fma, 3)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -446,6 +459,7 @@ def main():
                 cpu["value"] / (stage_ms["bbox"] + stage_ms["build"] + stage_ms["com"] + stage_ms["sort"]
                                 + stage_ms["force"]), 1)
 
+    probe = valu_probe()
     out = {
         "metric": "ms/step & G interactions/s, N=4M theta=0.5; batched & adjoint vs plain",
         "value": round(ginter, 2),
@@ -468,7 +482,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": force_traffic(n, args.theta),
-                     "kernel": "force_kernel", "flop_per_interaction": FLOP_PER_INTERACTION},
+                     "kernel": "force_kernel", "flop_per_interaction": FLOP_PER_INTERACTION, **probe},
         "cpu_baseline": cpu,
         "accuracy": accuracy,
     }

@@ -40,7 +40,7 @@ EXPORTS = [
     "nbx_sort_keys", "nbx_count_interactions", "nbx_step", "nbx_copy_buffer", "nbx_ic_star_disk",
     "nbx_ic_plummer", "nbx_ic_binary_disk", "nbx_autotune", "nbx_batch_create", "nbx_batch_destroy",
     "nbx_batch_last_error", "nbx_batch_bind", "nbx_batch_step", "nbx_batch_graph_step", "nbx_batch_check",
-    "nbx_batch_count_interactions", "nbx_batch_autotune", "nbx_batch_copy_buffer", "nbx_force_vjp",
+    "nbx_batch_count_interactions", "nbx_batch_autotune", "nbx_batch_copy_buffer", "nbx_force_vjp", "nbx_probe_valu",
 ]
 
 
@@ -100,6 +100,7 @@ def lib():
         getattr(L, name).argtypes = [i, C.c_uint64] + [vp] * 7
     ip = C.POINTER(C.c_int)
     L.nbx_autotune.argtypes = [vp, vp, ip]
+    L.nbx_probe_valu.argtypes = [i, i, i, vp, C.c_int64, vp, fp]
     L.nbx_batch_create.argtypes = [C.POINTER(Params), i, vp, vp, i, C.POINTER(vp)]
     L.nbx_batch_destroy.argtypes = [vp]
     L.nbx_batch_last_error.restype = C.c_char_p
@@ -136,6 +137,22 @@ def _stream(stream):
         import torch
         return torch.cuda.current_stream().cuda_stream
     return stream
+
+
+# ------------------------------------------------------------------ diagnostics
+def probe_valu(kind: str, blocks: int = 8192, iters: int = 4096, device: int = 0):
+    """Issue-rate probe of the FP32 vector ALU (nbx_probe_valu): kind "fma"
+    (v_fma_f32) or "rsq" (v_rsq_f32).  Returns (instructions per second per
+    lane-op, kernel ms, the per-lane results as a device tensor)."""
+    import torch
+    k = {"fma": 0, "rsq": 1}[kind]
+    out = torch.empty(256 * blocks, dtype=torch.float32, device=torch.device("cuda", device))
+    ms = C.c_float()
+    rc = lib().nbx_probe_valu(k, blocks, iters, C.c_void_p(out.data_ptr()), out.numel(), _stream(None), C.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"nbx_probe_valu failed ({rc})")
+    ops = 256.0 * blocks * iters * 8
+    return ops / (ms.value * 1e-3), ms.value, out
 
 
 # ------------------------------------------------------------------ initial conditions

@@ -69,6 +69,18 @@ def test_batch_create_validates_before_device():
     assert not h.value
 
 
+def test_valu_probe_validates_before_device():
+    L = nns_amd.lib()
+    ms = C.c_float()
+    buf = C.c_void_p(1)  # never dereferenced: every call below fails its checks first
+    assert L.nbx_probe_valu(2, 1, 1, buf, 256, None, C.byref(ms)) == -1      # kind
+    assert L.nbx_probe_valu(0, 0, 1, buf, 256, None, C.byref(ms)) == -1      # blocks
+    assert L.nbx_probe_valu(0, 1, 0, buf, 256, None, C.byref(ms)) == -1      # iters
+    assert L.nbx_probe_valu(0, 2, 1, buf, 511, None, C.byref(ms)) == -1      # 256 floats per block
+    assert L.nbx_probe_valu(0, 1, 1, None, 256, None, C.byref(ms)) == -1     # out
+    assert L.nbx_probe_valu(0, 1, 1, buf, 256, None, None) == -1             # ms
+
+
 def test_null_context_is_rejected():
     L = nns_amd.lib()
     assert L.nbx_check(None, None) == -1

@@ -419,3 +419,16 @@ def test_error_paths_do_not_fault(oracle):
     bh.update()
     bh.ctx.check()
     torch.cuda.synchronize()
+
+
+def test_valu_probe_chains_converge():
+    """The FP32 VALU probe behind the bench's measured issue rates runs both of
+    its chains to their fixed point (x -> 0.999 x + 1e-3 and x -> x^-1/2 both
+    end at 1, so each lane's sum of 8 registers is 8) and reports a time."""
+    _torch()
+    print("case valu probe", flush=True)
+    for kind, iters, tol in (("fma", 16384, 1e-3), ("rsq", 256, 1e-4)):
+        rate, ms, out = nns_amd.probe_valu(kind, blocks=512, iters=iters)
+        assert rate > 0 and ms > 0
+        v = out.cpu().numpy()
+        assert v.size == 512 * 256 and np.abs(v - 8.0).max() <= tol, (kind, np.abs(v - 8.0).max())
