@@ -73,6 +73,10 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: run `python -m nns_amd.build` (no CPU fallback)")
+    # torch first: libnbx.so then binds to the HIP runtime torch has loaded.
+    # Loaded the other way round, the process maps two HIP and two HSA
+    # runtimes, and the second one to start finds no GPU.
+    import torch  # noqa: F401
     L = C.CDLL(LIB_PATH)
     vp, i, f, fp = C.c_void_p, C.c_int, C.c_float, C.POINTER(C.c_float)
     L.nbx_default_params.argtypes = [C.POINTER(Params)]

@@ -5,6 +5,8 @@ statistics.  No kernel runs here."""
 import ctypes as C
 import os
 import re
+import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -27,6 +29,17 @@ def test_library_exports_every_declared_function():
     for name in declared:
         assert hasattr(lib, name), name
     assert sorted(nns_amd.EXPORTS) == declared
+
+
+def test_one_hip_runtime_whatever_the_import_order():
+    """Loading the library before anything else of torch still maps a single
+    HIP runtime (torch's): two in one process leave the second without a GPU."""
+    code = ("import nns_amd; nns_amd.lib(); import torch; "
+            "print(len({l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}))")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300,
+                       cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.strip().splitlines()[-1] == "1"
 
 
 def test_default_params():
