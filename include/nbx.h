@@ -257,7 +257,7 @@ int nbx_ic_binary_disk(int n, uint64_t seed, float *x, float *y, float *z, float
 
 /* Issue-rate probe of the FP32 vector ALU on the current device: `blocks` blocks of 256
  * threads, each lane `iters` rounds of 8 independent instructions, kind 0 = v_fma_f32
- * (2 flop each), kind 1 = v_rsq_f32.  out: device array of out_count >= 256 * blocks
+ * (2 flop each), kind 1 = v_rsq_f32; 1 <= blocks <= 2^22.  out: device array of out_count >= 256 * blocks
  * floats (one result per lane).  *ms receives the kernel time (hipEvents on `stream`;
  * synchronises it). */
 int nbx_probe_valu(int kind, int blocks, int iters, float *out, int64_t out_count, void *stream, float *ms);

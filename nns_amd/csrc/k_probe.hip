@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(kProbeThreads) valu_probe_kernel(int iters, fl
 extern "C" int nbx_probe_valu(int kind, int blocks, int iters, float *out, int64_t out_count, void *stream,
                               float *ms)
 {
-    if (kind < 0 || kind > 1 || blocks < 1 || iters < 1 || !out || !ms) return NBX_ERR_INVALID;
+    if (kind < 0 || kind > 1 || blocks < 1 || blocks > (1 << 22) || iters < 1 || !out || !ms) return NBX_ERR_INVALID;
     if ((int64_t)blocks * nbx::kProbeThreads > out_count) return NBX_ERR_INVALID;  // one float per thread
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t e0, e1;

@@ -88,6 +88,7 @@ def test_valu_probe_validates_before_device():
     buf = C.c_void_p(1)  # never dereferenced: every call below fails its checks first
     assert L.nbx_probe_valu(2, 1, 1, buf, 256, None, C.byref(ms)) == -1      # kind
     assert L.nbx_probe_valu(0, 0, 1, buf, 256, None, C.byref(ms)) == -1      # blocks
+    assert L.nbx_probe_valu(0, (1 << 22) + 1, 1, buf, 1 << 31, None, C.byref(ms)) == -1
     assert L.nbx_probe_valu(0, 1, 0, buf, 256, None, C.byref(ms)) == -1      # iters
     assert L.nbx_probe_valu(0, 2, 1, buf, 511, None, C.byref(ms)) == -1      # 256 floats per block
     assert L.nbx_probe_valu(0, 1, 1, None, 256, None, C.byref(ms)) == -1     # out
