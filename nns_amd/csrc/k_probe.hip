@@ -1,7 +1,7 @@
 // k_probe.hip -- issue-rate probes of the FP32 vector ALU (diagnostics).
 //
-// The force walk is bound by FP32 VALU issue.  These kernels measure the two
-// rates its ceiling is made of on the device at hand: v_fma_f32 (the scalar
+// The force walk's arithmetic ceiling is set by FP32 VALU issue.  These kernels
+// measure the two rates it is made of on the device at hand: v_fma_f32 (the scalar
 // FP32 FMA; packed FP32 is disabled for all device code of this library) and
 // v_rsq_f32.  Every lane runs `iters` rounds over 8 independent registers, so
 // each round is 8 instructions with no dependence inside it; one float per
