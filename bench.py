@@ -69,6 +69,8 @@ def parse():
                     help="bodies whose reference CPU tree walk is timed (0 = skip the CPU baseline)")
     ap.add_argument("--no-kahan", action="store_true")
     ap.add_argument("--skip", default="", help="comma list of sections to skip: plummer,batched,adjoint")
+    ap.add_argument("--batch-theta", type=float, default=None,
+                    help="one theta for every system of the batched section (default: 0.3/0.5/0.7 cycled)")
     ap.add_argument("--fd-coords", type=int, default=64, help="coordinates of the adjoint's central differences")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB copied once by a plain device copy before the timed region (profiler calibration)")
@@ -165,7 +167,12 @@ def run_batched(args):
     rng = np.random.default_rng(args.seed + 7)
     S = 512
     sizes = np.exp(rng.uniform(np.log(1024.0), np.log(65536.0), S)).astype(np.int32)
-    thetas = np.array([0.3, 0.5, 0.7] * (S // 3 + 1), np.float32)[:S]
+    if args.batch_theta is None:
+        thetas = np.array([0.3, 0.5, 0.7] * (S // 3 + 1), np.float32)[:S]
+        tdesc = "theta 0.3/0.5/0.7 cycled"
+    else:
+        thetas = np.full(S, args.batch_theta, np.float32)
+        tdesc = f"theta {args.batch_theta} for every system"
     parts = [nns_amd.initial_conditions("plummer", int(k), seed=1000 + s) for s, k in enumerate(sizes)]
     arrs = [np.concatenate([q[j] for q in parts]) for j in range(7)]
     del parts
@@ -213,8 +220,8 @@ def run_batched(args):
     ips_b = i_b / ms_b / 1e6
     ips_p = i_p / ms_p / 1e6
     names = ("reset", "bbox", "build", "com", "sort", "force", "update")
-    return {"workload": f"512 Plummer spheres, N_s log-uniform in [1024, 65536] (total {N}), theta 0.3/0.5/0.7 "
-                        f"cycled; plain: one Plummer sphere N={N}, theta=0.5",
+    return {"workload": f"512 Plummer spheres, N_s log-uniform in [1024, 65536] (total {N}), {tdesc}; "
+                        f"plain: one Plummer sphere N={N}, theta=0.5",
             "systems": S, "total_n": N, "force_waves": waves,
             "batched_ms_per_step": round(ms_b, 4), "batched_interactions_per_step": int(i_b),
             "batched_g_interactions_per_s": round(ips_b, 2),
