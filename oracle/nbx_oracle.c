@@ -30,6 +30,7 @@
  *                        (Interaction.cpp:73-96) and the SI-unit force of
  *                        singleInteraction (Interaction.cpp:17-55).
  *   orc_direct_accel     FP64 direct sum, the closed form a_i = sum_j m_j d_ij (|d_ij|^2+eps^2)^-3/2.
+ *   orc_direct_accel_fast the same sum vectorised over sources (summation order differs).
  *   orc_update           symplectic Euler of updateKernel (Kernels.cu:581-600), written with
  *                        explicit fused multiply-adds so it is bit-reproducible.
  *
@@ -592,17 +593,23 @@ static void direct_generic(
     ORC_DIRECT_LOOP
 }
 
-void orc_direct_accel_fast(int64_t n, const float *x, const float *y, const float *z, const float *m,
-                           double eps2, int64_t ntarget, const int32_t *targets, double *acc)
+/* Returns 0, or -1 when the FP64 copies cannot be allocated (acc untouched). */
+int orc_direct_accel_fast(int64_t n, const float *x, const float *y, const float *z, const float *m,
+                          double eps2, int64_t ntarget, const int32_t *targets, double *acc)
 {
-    double *X = malloc(8 * (size_t)n), *Y = malloc(8 * (size_t)n), *Z = malloc(8 * (size_t)n),
-           *M = malloc(8 * (size_t)n);
+    const size_t bytes = 8 * (size_t)(n > 0 ? n : 1);
+    double *X = malloc(bytes), *Y = malloc(bytes), *Z = malloc(bytes), *M = malloc(bytes);
+    if (!X || !Y || !Z || !M) {
+        free(X); free(Y); free(Z); free(M);
+        return -1;
+    }
     for (int64_t j = 0; j < n; ++j) { X[j] = x[j]; Y[j] = y[j]; Z[j] = z[j]; M[j] = m[j]; }
     if (__builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma"))
         direct_avx2(n, x, y, z, X, Y, Z, M, eps2, ntarget, targets, acc);
     else
         direct_generic(n, x, y, z, X, Y, Z, M, eps2, ntarget, targets, acc);
     free(X); free(Y); free(Z); free(M);
+    return 0;
 }
 
 /* ------------------------------------------------------------- update */

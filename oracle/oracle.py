@@ -72,6 +72,7 @@ def lib():
                                    C.c_void_p, _f64p]
     L.orc_tree_vjp.restype = C.c_int
     L.orc_tree_vjp.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _f64p, _f64p, _f64p]
+    L.orc_direct_accel_fast.restype = C.c_int
     L.orc_direct_accel_fast.argtypes = [C.c_int64, _f32p, _f32p, _f32p, _f32p, C.c_double, C.c_int64,
                                         C.c_void_p, _f64p]
     L.orc_update.argtypes = [C.c_int64] + [_f32p] * 9 + [C.c_float, C.c_float]
@@ -213,7 +214,8 @@ def direct_accel_fast(x, y, z, m, eps2, targets=None):
         targets = np.ascontiguousarray(targets, dtype=np.int32)
         k, tp = len(targets), targets.ctypes.data_as(C.c_void_p)
     acc = np.zeros(3 * k, np.float64)
-    lib().orc_direct_accel_fast(len(x), x, y, z, m, float(eps2), k, tp, acc)
+    if lib().orc_direct_accel_fast(len(x), x, y, z, m, float(eps2), k, tp, acc) != 0:
+        raise MemoryError("orc_direct_accel_fast: allocation of the FP64 source copies failed")
     return acc.reshape(k, 3)
 
 

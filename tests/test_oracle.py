@@ -245,6 +245,95 @@ def test_direct_two_body_and_cube(oracle):
     assert np.abs(a[8]).max() < 1e-15
 
 
+def _abs_term_sum(x, y, z, m, eps2):
+    """sum_j |m_j d_ij| (|d_ij|^2 + eps^2)^-3/2 for every body i, in numpy FP64:
+    the size of the direct sum without cancellation, the scale that the rounding
+    of any summation order is bounded by."""
+    p = np.stack([x, y, z], 1).astype(np.float64)
+    mm = m.astype(np.float64)
+    out = np.zeros(len(x))
+    for lo in range(0, len(x), 500):
+        d = p[None, :, :] - p[lo:lo + 500, None, :]
+        d2 = (d * d).sum(-1)
+        w = np.where(d2 > 0, mm[None, :] / np.where(d2 > 0, (d2 + eps2) ** 1.5, 1.0), 0.0)
+        out[lo:lo + 500] = (w * np.sqrt(d2)).sum(1)
+    return out
+
+
+@pytest.mark.parametrize("eps2", [0.0, 1e-6])
+def test_direct_fast_equals_direct(oracle, eps2):
+    """orc_direct_accel_fast, the FP64 reference of the bench's accuracy figures,
+    sums the closed form of orc_direct_accel, possibly in another order (it may
+    be vectorised over sources).  With coincident bodies (left out at zero distance, also when
+    eps = 0) the two agree to the rounding of the summation:
+    |a_fast - a| <= 1e-13 * sum_j |term_j| for every body; a target list gives
+    the same values bit for bit as the full call."""
+    r = np.random.default_rng(31)
+    n = 4000
+    p = r.normal(0, 1, (n, 3))
+    p[100:141] = p[0]      # 42 bodies on one point
+    p[200:203] = p[7]      # and 4 on another
+    x, y, z = (p[:, k].astype(np.float32) for k in range(3))
+    m = r.uniform(0.5, 1.5, n).astype(np.float32) / n
+    a = oracle.direct_accel(x, y, z, m, eps2)
+    af = oracle.direct_accel_fast(x, y, z, m, eps2)
+    assert np.isfinite(af).all()
+    scale = _abs_term_sum(x, y, z, m, eps2)
+    err = np.linalg.norm(af - a, axis=1)
+    assert (err <= 1e-13 * scale).all(), (err / scale).max()
+    tg = np.sort(np.concatenate([[0, 7, 100, 140, 200], r.choice(n, 300, replace=False)])).astype(np.int32)
+    assert np.array_equal(oracle.direct_accel_fast(x, y, z, m, eps2, tg), af[tg])
+
+
+def test_direct_fast_closed_forms(oracle):
+    """Two bodies and the centre of a cube, as for the scalar direct sum."""
+    eps = 0.1
+    x = np.array([0.0, 2.0], np.float32); y = np.zeros(2, np.float32); z = np.zeros(2, np.float32)
+    m = np.array([1.0, 3.0], np.float32)
+    a = oracle.direct_accel_fast(x, y, z, m, eps * eps)
+    assert a[0, 0] == pytest.approx(3.0 * 2.0 / (4.0 + eps * eps) ** 1.5, rel=1e-14)
+    assert a[1, 0] == pytest.approx(-1.0 * 2.0 / (4.0 + eps * eps) ** 1.5, rel=1e-14)
+    assert a[0, 1] == 0.0 and a[1, 2] == 0.0
+    c = np.array([[i, j, k] for i in (-1, 1) for j in (-1, 1) for k in (-1, 1)] + [[0, 0, 0]], np.float32)
+    a = oracle.direct_accel_fast(c[:, 0], c[:, 1], c[:, 2], np.ones(9, np.float32), 0.0)
+    assert np.abs(a[8]).max() < 1e-15
+
+
+def _span_reference(xf, yf, zf):
+    """Interaction.cpp:143-147 on the float inputs promoted to double:
+    xSpan = |xMax - xMin| (and y, z); maxSpan = xSpan > ySpan ? xSpan : ySpan;
+    maxSpan = maxSpan < zSpan ? zSpan : maxSpan."""
+    lo = [float(v.min()) for v in (xf, yf, zf)]
+    hi = [float(v.max()) for v in (xf, yf, zf)]
+    xs, ys, zs = (abs(h - l) for h, l in zip(hi, lo))
+    ms = xs if xs > ys else ys
+    ms = zs if ms < zs else ms
+    return lo, ms
+
+
+@pytest.mark.parametrize("case", KEY_CASES + ["long_x", "long_y", "long_z"])
+def test_bbox_is_min_max_and_span(oracle, case):
+    """orc_bbox: the float min/max of each axis exactly, and the FP64 extent
+    (origin = the minima, maxSpan by the reference's formula) exactly.  Clouds
+    stretched along each axis in turn make each axis's span the one chosen."""
+    if case.startswith("long_"):
+        r = np.random.default_rng(41)
+        p = r.normal(0, 1, (3000, 3)) + 5.0
+        p[:, "xyz".index(case[-1])] *= 7.0
+        x, y, z = (p[:, k].astype(np.float32) for k in range(3))
+    else:
+        g = golden(f"keys_{case}.npz")
+        x, y, z = g["x"], g["y"], g["z"]
+    mm, ext = oracle.bbox(x, y, z)
+    assert np.array_equal(mm, np.array([x.min(), x.max(), y.min(), y.max(), z.min(), z.max()], np.float32))
+    lo, span = _span_reference(x, y, z)
+    assert ext[0] == lo[0] and ext[1] == lo[1] and ext[2] == lo[2]
+    assert ext[3] == span
+    if case.startswith("long_"):
+        k = "xyz".index(case[-1])
+        assert span == float(mm[2 * k + 1]) - float(mm[2 * k])
+
+
 def test_bh_two_cluster_monopole(oracle):
     """A tight cluster far away is accepted as one monopole: the force is the
     cluster's total mass at its centre of mass."""
