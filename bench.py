@@ -77,16 +77,39 @@ def parse():
     return ap.parse_args()
 
 
+def host_cpu():
+    """(nproc, usable cores of this process, CPU model) of the host."""
+    nproc = os.cpu_count() or 1
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return nproc, usable, model
+
+
+CPU_REPS = 3  # timed CPU steps; the baseline is their median
+
+
 def cpu_baseline(x, y, z, m, theta, softening, sample):
     """Times the reference's serial Tree::insert of all bodies and its OpenMP
     treeInteraction loop over `sample` bodies (oracle/_ref/ref_harness), with the
     bodies walked in key order (the reference's `hashed` path, the faster of its
-    two) and, beside it, in index order."""
+    two) and, beside it, in index order; CPU_REPS steps each, median reported.
+    Threads: OMP_NUM_THREADS when the host sets it (the CPU share a shared GPU
+    host grants this job), else every core this process may run on."""
     from oracle import oracle as O
     if not O.have_ref():
         return None
     n = len(x)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    nproc, usable, model = host_cpu()
+    preset = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    threads = preset or usable
     recs = {}
     with tempfile.TemporaryDirectory() as td:
         fin = os.path.join(td, "bodies.bin")
@@ -97,7 +120,8 @@ def cpu_baseline(x, y, z, m, theta, softening, sample):
         env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
         for hashed in (1, 0):
             r = subprocess.run([O.REF_HARNESS, "time", fin, repr(theta), repr(softening * O.TO_METERS),
-                                str(sample), str(hashed)], capture_output=True, text=True, env=env, timeout=900)
+                                str(sample), str(hashed), str(CPU_REPS)], capture_output=True, text=True, env=env,
+                               timeout=900)
             if r.returncode != 0:
                 return {"error": r.stderr[-400:]}
             recs[hashed] = json.loads(r.stdout.strip().splitlines()[-1])
@@ -107,14 +131,19 @@ def cpu_baseline(x, y, z, m, theta, softening, sample):
 
     rec, idx = recs[1], recs[0]
     force_full = rec["force_sample_s"] * n / rec["nsample"]
+    share = (f"OMP_NUM_THREADS={preset} set by the host (its CPU share for this job)" if preset
+             else "every usable core")
     return {"value": round(ms_of(rec), 1), "unit": "ms/step (tree build + force)", "cores": rec["threads"],
             "kind": "reference",
-            "sample": (f"reference Tree::insert of all {n} bodies ({rec['build_s']:.2f} s, serial) + "
-                       f"Interaction::treeInteraction over every {n // rec['nsample']}-th body in key order "
-                       f"(the reference's hashed path; {rec['nsample']} bodies, {rec['force_sample_s']:.2f} s "
-                       f"on {rec['threads']} threads), force scaled by n/sample; the key map "
-                       f"({rec['key_s']:.2f} s) is outside the reference's timers and not counted"),
+            "sample": (f"median of {CPU_REPS} steps: reference Tree::insert of all {n} bodies "
+                       f"({rec['build_s']:.2f} s, serial) + Interaction::treeInteraction over every "
+                       f"{n // rec['nsample']}-th body in key order (the reference's hashed path; {rec['nsample']} "
+                       f"bodies, {rec['force_sample_s']:.2f} s on {rec['threads']} threads, {share}), force scaled "
+                       f"by n/sample; the key map ({rec['key_s']:.2f} s) is outside the reference's timers and not "
+                       f"counted"),
+            "nproc": nproc, "usable_cores": usable, "cpu_model": model, "steps": CPU_REPS,
             "build_s": rec["build_s"], "force_s_extrapolated": round(force_full, 3),
+            "build_reps_s": rec.get("build_reps_s"), "force_sample_reps_s": rec.get("force_reps_s"),
             "index_order": {"value": round(ms_of(idx), 1), "force_sample_s": idx["force_sample_s"],
                             "build_s": idx["build_s"]}}
 
@@ -389,6 +418,10 @@ def main():
     torch.cuda.synchronize()
     i_alg1 = int(counts1.sum().item())
     i_alg = 0.5 * (i_alg0 + i_alg1)
+    try:  # the work the walk did for those interactions (lane slots, frames)
+        walk = bh.walk_stats()
+    except nns_amd.NbxError:
+        walk = None
     ms_per_step = total_ms / K
     t_force = stage_ms["force"] * 1e-3
     ginter = i_alg / t_force / 1e9
@@ -489,7 +522,14 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": force_traffic(n, args.theta),
-                     "kernel": "force_kernel", "flop_per_interaction": FLOP_PER_INTERACTION, **probe},
+                     "kernel": "force_ring_kernel" if walk else "force_kernel",
+                     "flop_per_interaction": FLOP_PER_INTERACTION, **probe,
+                     # counted on the device by the walk itself: one child step tests one child for
+                     # the 64 lanes of a group; useful = accepted interactions / (64 x steps)
+                     **({"useful_lane_fraction": round(walk["useful_lane_fraction"], 4),
+                         "work_inflation": round(walk["work_inflation"], 3),
+                         "child_steps_per_group": round(walk["child_steps"] / walk["groups"], 1),
+                         "frames_per_group": round(walk["frames"] / walk["groups"], 1)} if walk else {})},
         "cpu_baseline": cpu,
         "accuracy": accuracy,
     }

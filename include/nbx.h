@@ -28,8 +28,9 @@
  *   - stages return an int status (NBX_OK or a negative NBX_ERR_*) instead of
  *     nothing; errors raised inside kernels (tree capacity, bucket overflow)
  *     are latched in the context and reported by nbx_check; once latched, the
- *     device stages that follow on that context do no work until the next
- *     nbx_reset_arrays;
+ *     device stages that follow on that context, nbx_update included (the
+ *     bodies stay where they are), do no work until the next nbx_reset_arrays,
+ *     which keeps the latched error for nbx_check to report;
  *   - theta, softening, G, dt and d are run-time parameters (nbx_params); the
  *     reference hard-codes theta = 1.5 and eps^2 = 0.0025 (src/Kernels.cu:17-18)
  *     and ignores its `g` argument;
@@ -77,7 +78,8 @@ typedef struct nbx_params {
     int kahan;        /* 1: compensated (Kahan) accumulation of the accelerations       */
     int block_size;   /* threads per block of the update stage: 0 (= 256) or a multiple  */
                       /* of 64 in [64, 1024] (the reference's blockSize, main.cu:30)   */
-    int force_waves;  /* waves per block of the force kernel (0 = 4)                    */
+    int force_waves;  /* waves per block of the force kernel: 2, 4 or 8; 0 = 4 in
+                         nbx_create, "keep the current choice" in nbx_set_params        */
     int build_locked; /* 1: subtrees built by lock-based insertion (compare-and-swap on
                          child slots, Kernels.cu:138-241); 0: from the shared-digit
                          counts of neighbouring sorted keys, one store per slot.  Both
@@ -99,15 +101,17 @@ int nbx_set_params(nbx_context *ctx, const nbx_params *p);
 int nbx_get_params(const nbx_context *ctx, nbx_params *p);
 /* Message of the last error on this context ("" if none).  ctx may be NULL for nbx_create errors. */
 const char *nbx_last_error(const nbx_context *ctx);
-/* Waits for the stream, then returns NBX_OK or the first device-side error latched
- * since the last nbx_reset_arrays. */
+/* Waits for the stream, then returns NBX_OK or a device-side error latched since the
+ * last nbx_check: an error of an earlier step is kept by nbx_reset_arrays and
+ * reported once here, so a caller may check once after many steps. */
 int nbx_check(nbx_context *ctx, void *stream);
 
 /* ---- the seven stages (KernelsWrapper.cuh:28-48) ------------------------------ */
 
 /* resetArrays (KernelsWrapper.cu:80-106, Kernels.cu:21-64): child = -1; count = 1 for
  * bodies, 0 for cells; cell x,y,z,mass = 0; start = -1; sorted = 0; *index = n;
- * *mutex = 0; bounding box = 0; clears the context's error latch. */
+ * *mutex = 0; bounding box = 0; clears the context's error latch (an error latched
+ * there stays reported by nbx_check). */
 int nbx_reset_arrays(nbx_context *ctx, int *mutex, float *x, float *y, float *z, float *mass,
                      int *count, int *start, int *sorted, int *child, int *index, float *minX,
                      float *maxX, float *minY, float *maxY, float *minZ, float *maxZ, int n, int m,
@@ -120,7 +124,8 @@ int nbx_compute_bounding_box(nbx_context *ctx, int *mutex, float *x, float *y, f
                              float *minX, float *maxX, float *minY, float *maxY, float *minZ,
                              float *maxZ, int n, void *stream, float *elapsed_ms);
 
-/* buildTree (KernelsWrapper.cu:136-162, Kernels.cu:157-377): Morton keys, stable radix
+/* buildTree (KernelsWrapper.cu:136-162, Kernels.cu:157-377): x, y, z and n must be those of
+ * the last nbx_compute_bounding_box (NBX_ERR_INVALID otherwise).  Morton keys, stable radix
  * sort of (key, body), then lock-based insertion along the key digits (child slots
  * locked with -2 by compare-and-swap).  Fills child and *index.  Cell sums are left to
  * nbx_centre_of_mass. */
@@ -148,14 +153,16 @@ int nbx_compute_forces(nbx_context *ctx, float *x, float *y, float *z, float *vx
                        int *child, float *minX, float *maxX, int n, float g, void *stream,
                        float *elapsed_ms);
 
-/* update (KernelsWrapper.cu:208, Kernels.cu:581-600): v = fma(dt, a, v); x = fma(d*dt, v, x). */
+/* update (KernelsWrapper.cu:208, Kernels.cu:581-600): v = fma(dt, a, v); x = fma(d*dt, v, x).
+ * Does nothing while an error of this step is latched (see nbx_check). */
 int nbx_update(nbx_context *ctx, float *x, float *y, float *z, float *vx, float *vy, float *vz,
                float *ax, float *ay, float *az, int n, float dt, float d, void *stream,
                float *elapsed_ms);
 
 /* ---- extensions (no counterpart in KernelsWrapper) ------------------------------ */
 
-/* Keys of n bodies for the extent found by the last nbx_compute_bounding_box:
+/* Keys of n bodies for the extent found by the last nbx_compute_bounding_box (over the
+ * same x, y, z and n, else NBX_ERR_INVALID):
  * NBX_KEY_MORTON is Body::getKey (3D/OpenMP/OpenMP_NBody/src/Body.cpp:13-32);
  * NBX_KEY_HILBERT is Lebesgue2Hilbert of its 63 key bits
  * (MolecularDynamics/BarnesHutParallel/SpaceFillingCurves/src/Tree.cpp:82-104). */
@@ -167,18 +174,48 @@ int nbx_compute_keys(nbx_context *ctx, const float *x, const float *y, const flo
 int nbx_sort_keys(nbx_context *ctx, const uint64_t *keys, int n, uint64_t *sorted_keys,
                   int32_t *perm, void *stream, float *elapsed_ms);
 
+/* Stars ("suns") outside the tree: the reference's OpenMP step keeps its suns apart from
+ * the bodies it inserts into the octree and couples them directly, both ways
+ * (3D/OpenMP/OpenMP_NBody/src/Interaction.cpp:111-124; updated with the bodies,
+ * :219-234).  This stage adds to each of the n bodies' ax, ay, az (call it after
+ * nbx_compute_forces, which writes them) the pull of the ns stars, and writes the
+ * stars' own accelerations sax, say, saz: the pull of every other star and of every
+ * body.  Plummer kernel G m d (|d|^2 + eps^2)^-3/2 with the context's softening; zero
+ * distances are left out.  The terms are summed in FP64 in a fixed order (the same
+ * result on every run).  0 <= ns <= NBX_MAX_STARS; the stars' arrays are device
+ * arrays of ns floats.  Does nothing while an error of this step is latched. */
+#define NBX_MAX_STARS 8
+int nbx_star_forces(nbx_context *ctx, const float *sx, const float *sy, const float *sz, const float *smass,
+                    float *sax, float *say, float *saz, int ns, const float *x, const float *y, const float *z,
+                    const float *mass, float *ax, float *ay, float *az, int n, float g, void *stream,
+                    float *elapsed_ms);
+
 /* Per-body interaction counts of the last tree (cells accepted plus leaf bodies other
  * than the body itself), written to counts[i] for body i.  Not part of a time step. */
 int nbx_count_interactions(nbx_context *ctx, int n, int64_t *counts, void *stream);
 
+/* Work of the force walk over the last tree (not part of a time step): out[0] the
+ * interactions (the sum of the per-particle lists, as nbx_count_interactions),
+ * out[1] the child steps (one step tests one child for the 64 lanes of a group, so
+ * 64 out[1] lane slots hold out[0] useful ones), out[2] the frames (opened cells
+ * walked), out[3] the groups of 64 bodies.  The reference's warp-vote walk pays the
+ * same way for the union of a warp's lists (Kernels.cu:543). */
+int nbx_walk_stats(nbx_context *ctx, int64_t *out, void *stream);
+
 /* One full step (reset, bounding box, build, centre of mass, sort, forces, update) over
  * the arrays of the reference layout; stage_ms (7 floats, nullable) receives per-stage
- * hipEvent times without host waits between stages. */
+ * hipEvent times without host waits between stages.  With nstars > 0 the step is the
+ * reference's OpenMP step with suns (Interaction::interactBodies): nbx_star_forces runs
+ * after the tree forces (its time counts in the force stage) and the stars are updated
+ * with the bodies (in the update stage). */
 typedef struct nbx_arrays {
     float *x, *y, *z, *mass, *vx, *vy, *vz, *ax, *ay, *az;
     int *count, *start, *sorted, *child, *index, *mutex;
     float *minX, *maxX, *minY, *maxY, *minZ, *maxZ;
     int n, m;
+    /* stars outside the tree: nstars (0..NBX_MAX_STARS) device arrays of each */
+    float *sx, *sy, *sz, *smass, *svx, *svy, *svz, *sax, *say, *saz;
+    int nstars;
 } nbx_arrays;
 int nbx_step(nbx_context *ctx, const nbx_arrays *a, void *stream, float *stage_ms);
 
@@ -237,6 +274,8 @@ int nbx_batch_graph_step(nbx_batch *b, void *stream);
 int nbx_batch_check(nbx_batch *b, void *stream);
 /* Per-body interaction counts of the last tree, device array of N int64. */
 int nbx_batch_count_interactions(nbx_batch *b, int64_t *counts, void *stream);
+/* nbx_walk_stats of the ensemble's last tree (groups: the force table's groups). */
+int nbx_batch_walk_stats(nbx_batch *b, int64_t *out, void *stream);
 /* nbx_autotune for the ensemble's force kernel (invalidates a captured graph). */
 int nbx_batch_autotune(nbx_batch *b, void *stream, int *waves_out);
 /* Host copy of a batch buffer: NBX_BUF_KEYS, NBX_BUF_SORTED_KEYS, NBX_BUF_PERM (N each),

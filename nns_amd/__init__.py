@@ -41,7 +41,15 @@ EXPORTS = [
     "nbx_ic_plummer", "nbx_ic_binary_disk", "nbx_autotune", "nbx_batch_create", "nbx_batch_destroy",
     "nbx_batch_last_error", "nbx_batch_bind", "nbx_batch_step", "nbx_batch_graph_step", "nbx_batch_check",
     "nbx_batch_count_interactions", "nbx_batch_autotune", "nbx_batch_copy_buffer", "nbx_force_vjp", "nbx_probe_valu",
+    "nbx_star_forces", "nbx_walk_stats", "nbx_batch_walk_stats",
 ]
+
+
+def _walk_dict(v):
+    inter, steps, frames, groups = (int(x) for x in v)
+    return {"interactions": inter, "child_steps": steps, "frames": frames, "groups": groups,
+            "useful_lane_fraction": inter / (64.0 * steps) if steps else 0.0,
+            "work_inflation": 64.0 * steps / inter if inter else 0.0}
 
 
 class NbxError(RuntimeError):
@@ -56,11 +64,15 @@ class Params(C.Structure):
                 ("block_size", C.c_int), ("force_waves", C.c_int), ("build_locked", C.c_int)]
 
 
+STAR_KEYS = ("sx", "sy", "sz", "smass", "svx", "svy", "svz", "sax", "say", "saz")
+MAX_STARS = 8  # NBX_MAX_STARS
+
+
 class Arrays(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("x", "y", "z", "mass", "vx", "vy", "vz", "ax", "ay", "az",
                                           "count", "start", "sorted", "child", "index", "mutex",
                                           "minX", "maxX", "minY", "maxY", "minZ", "maxZ")] + \
-               [("n", C.c_int), ("m", C.c_int)]
+               [("n", C.c_int), ("m", C.c_int)] + [(k, C.c_void_p) for k in STAR_KEYS] + [("nstars", C.c_int)]
 
 
 _lib = None
@@ -97,6 +109,9 @@ def lib():
     L.nbx_compute_keys.argtypes = [vp, vp, vp, vp, i, i, vp, vp, fp]
     L.nbx_sort_keys.argtypes = [vp, vp, i, vp, vp, vp, fp]
     L.nbx_count_interactions.argtypes = [vp, i, vp, vp]
+    L.nbx_star_forces.argtypes = [vp] + [vp] * 7 + [i] + [vp] * 7 + [i, f, vp, fp]
+    L.nbx_walk_stats.argtypes = [vp, vp, vp]
+    L.nbx_batch_walk_stats.argtypes = [vp, vp, vp]
     L.nbx_force_vjp.argtypes = [vp, vp, vp, vp, f, vp, vp, vp, vp, vp, fp]
     L.nbx_step.argtypes = [vp, C.POINTER(Arrays), vp, fp]
     L.nbx_copy_buffer.argtypes = [vp, i, vp, C.c_int64, vp]
@@ -204,6 +219,12 @@ class Context:
         self.params = params
         self._rc(lib().nbx_set_params(self.h, C.byref(params)))
 
+    def get_params(self) -> Params:
+        """The context's parameters (force_waves: the kernel shape in use)."""
+        p = Params()
+        self._rc(lib().nbx_get_params(self.h, C.byref(p)))
+        return p
+
     def check(self, stream=None):
         self._rc(lib().nbx_check(self.h, _stream(stream)))
 
@@ -270,6 +291,13 @@ class KernelsWrapper:
     def sortKeys(self, keys, n, sorted_keys, perm, timing=False):
         return self._call(lib().nbx_sort_keys, _ptr(keys), int(n), _ptr(sorted_keys), _ptr(perm), timing=timing)
 
+    def starForces(self, sx, sy, sz, smass, sax, say, saz, ns, x, y, z, mass, ax, ay, az, n, g, timing=False):
+        """Stars outside the tree (nbx_star_forces; the reference's suns,
+        3D/OpenMP/OpenMP_NBody/src/Interaction.cpp:111-124)."""
+        a = [_ptr(t) for t in (sx, sy, sz, smass, sax, say, saz)]
+        b = [_ptr(t) for t in (x, y, z, mass, ax, ay, az)]
+        return self._call(lib().nbx_star_forces, *a, int(ns), *b, int(n), float(g), timing=timing)
+
 
 STAGES = ("reset", "bbox", "build", "com", "sort", "force", "update")
 
@@ -279,7 +307,12 @@ class BarnesHut:
     step of BarnesHut::update (src/BarnesHut.cu:195-301), without the per-step
     device-to-host copy.  numNodes m = 2N + 12000 unless given."""
 
-    def __init__(self, x, y, z, vx, vy, vz, mass, params: Params | None = None, m=None, device=0, ctx=None):
+    def __init__(self, x, y, z, vx, vy, vz, mass, params: Params | None = None, m=None, device=0, ctx=None,
+                 stars=None):
+        """stars: optional (x, y, z, vx, vy, vz, mass) host arrays of up to
+        MAX_STARS bodies kept outside the tree, the reference's suns
+        (3D/OpenMP/OpenMP_NBody/src/Interaction.cpp:111-124); update() then
+        couples them to the bodies directly and moves them with the bodies."""
         import torch
         n = len(x)
         self.n = n
@@ -301,7 +334,15 @@ class BarnesHut:
             self.arr[k] = torch.zeros(1, **i32)
         for k in ("minX", "maxX", "minY", "maxY", "minZ", "maxZ"):
             self.arr[k] = torch.zeros(1, **f32)
-        self._arrays = Arrays(**{k: v.data_ptr() for k, v in self.arr.items()}, n=n, m=self.m)
+        self.nstars = 0 if stars is None else len(stars[0])
+        if self.nstars > MAX_STARS:
+            raise ValueError(f"at most {MAX_STARS} stars")
+        if self.nstars:
+            for k, v in zip(("sx", "sy", "sz", "svx", "svy", "svz", "smass"), stars):
+                self.arr[k] = torch.as_tensor(np.asarray(v, np.float32), device=self.dev).contiguous()
+            for k in ("sax", "say", "saz"):
+                self.arr[k] = torch.zeros(self.nstars, **f32)
+        self._arrays = Arrays(**{k: v.data_ptr() for k, v in self.arr.items()}, n=n, m=self.m, nstars=self.nstars)
         self.step_count = 0
 
     def __getattr__(self, k):
@@ -344,6 +385,13 @@ class BarnesHut:
         self.ctx._rc(lib().nbx_count_interactions(self.ctx.h, self.n, out.data_ptr(), _stream(None)))
         return out
 
+    def walk_stats(self):
+        """Work of the force walk on the last tree (nbx_walk_stats): interactions,
+        child steps, frames, groups, and the useful fraction of the lane slots."""
+        v = (C.c_int64 * 4)()
+        self.ctx._rc(lib().nbx_walk_stats(self.ctx.h, v, _stream(None)))
+        return _walk_dict(v)
+
     def rebuild(self, timing=False):
         """Reset, bounding box, build, centre of mass and sort of the current
         positions (the tree that nbx_compute_forces and nbx_force_vjp walk)."""
@@ -380,7 +428,9 @@ class BarnesHut:
 
     def host(self, k, count=None):
         t = self.arr[k]
-        return t[: (count if count is not None else self.n)].cpu().numpy()
+        if count is None:
+            count = self.nstars if k in STAR_KEYS else self.n
+        return t[:count].cpu().numpy()
 
 
 class Batch:
@@ -448,6 +498,11 @@ class Batch:
         out = torch.zeros(self.N, dtype=torch.int64, device=self.dev)
         self._rc(lib().nbx_batch_count_interactions(self.h, out.data_ptr(), _stream(None)))
         return out
+
+    def walk_stats(self):
+        v = (C.c_int64 * 4)()
+        self._rc(lib().nbx_batch_walk_stats(self.h, v, _stream(None)))
+        return _walk_dict(v)
 
     def copy_buffer(self, which, count, dtype):
         out = np.zeros(count, dtype)

@@ -29,7 +29,8 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-I", os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["k_basic.hip", "k_radix.hip", "k_tree.hip", "k_force.hip", "k_adjoint.hip", "nbx_api.cpp", "nbx_batch.cpp", "nbx_ics.cpp", "k_probe.hip"]
+SOURCES = ["k_basic.hip", "k_radix.hip", "k_tree.hip", "k_force.hip", "k_adjoint.hip", "nbx_api.cpp", "nbx_batch.cpp",
+           "nbx_ics.cpp", "k_probe.hip"]
 
 
 def _compile(src: str) -> str:

@@ -32,6 +32,8 @@ __global__ void __launch_bounds__(256) reset_kernel(
         *mutex = 0;
         *index = n;
         *minX = 0.f; *maxX = 0.f; *minY = 0.f; *maxY = 0.f; *minZ = 0.f; *maxZ = 0.f;
+        st->sticky |= st->err & ~st->seen;  // an unreported error of this step waits for nbx_check
+        st->seen = 0;
         st->err = 0;
         st->bbox_ticket = 0u;
     }
@@ -149,11 +151,14 @@ __global__ void __launch_bounds__(256) keys_kernel(
 
 // updateKernel (Kernels.cu:581-600): v += dt a; x += d dt v, with explicit fused
 // multiply-adds so that the result does not depend on compiler contraction.
+// After an error of this step (latch raised) the bodies are left where they
+// are: the accelerations were not computed.
 __global__ void __launch_bounds__(1024) update_kernel(
     float *__restrict__ x, float *__restrict__ y, float *__restrict__ z, float *__restrict__ vx,
     float *__restrict__ vy, float *__restrict__ vz, const float *__restrict__ ax,
-    const float *__restrict__ ay, const float *__restrict__ az, int n, float dt, float d)
+    const float *__restrict__ ay, const float *__restrict__ az, int n, float dt, float d, const DeviceState *st)
 {
+    if (st && latched(st)) return;
     const float ddt = d * dt;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += blockDim.x * gridDim.x) {
         float u = fmaf(dt, ax[i], vx[i]);
@@ -164,6 +169,102 @@ __global__ void __launch_bounds__(1024) update_kernel(
         y[i] = fmaf(ddt, v, y[i]);
         z[i] = fmaf(ddt, w, z[i]);
     }
+}
+
+// ------------------------------------------------------------ stars ("suns")
+// Stars are bodies outside the tree (3D/OpenMP/OpenMP_NBody/src/
+// Interaction.cpp:111-124): each star pulls every body and every other star
+// directly and is pulled back by every body, with the Plummer kernel
+// m d (|d|^2 + eps^2)^-3/2; a zero distance contributes nothing (:28-30).
+// The terms are evaluated in FP64.  Bodies: one thread each, the sum over the
+// stars added once to the FP32 acceleration the tree walk wrote.  Stars: each
+// block reduces its bodies' pulls per star (wave shuffles, then LDS) to one
+// partial per (block, star); star_final_kernel adds the partials in block
+// order and the star-star terms: no atomics, the same sums on every run.
+constexpr int kStarThreads = 256;
+
+__device__ inline double wave_sum_f64(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__global__ void __launch_bounds__(kStarThreads) star_bodies_kernel(
+    const float *__restrict__ sx, const float *__restrict__ sy, const float *__restrict__ sz,
+    const float *__restrict__ sm, int ns, const float *__restrict__ x, const float *__restrict__ y,
+    const float *__restrict__ z, const float *__restrict__ mass, float *__restrict__ ax, float *__restrict__ ay,
+    float *__restrict__ az, int n, double G, double eps2, double *__restrict__ part, const DeviceState *st)
+{
+    __shared__ double s_red[kStarThreads / 64][NBX_MAX_STARS][3];
+    if (latched(st)) return;  // one value for the whole launch: nothing in it raises the latch
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double qx[NBX_MAX_STARS], qy[NBX_MAX_STARS], qz[NBX_MAX_STARS], qm[NBX_MAX_STARS];
+    double px[NBX_MAX_STARS], py[NBX_MAX_STARS], pz[NBX_MAX_STARS];  // this thread's pulls on each star
+#pragma unroll
+    for (int s = 0; s < NBX_MAX_STARS; ++s) {
+        const bool on = s < ns;
+        qx[s] = on ? sx[s] : 0.0; qy[s] = on ? sy[s] : 0.0; qz[s] = on ? sz[s] : 0.0; qm[s] = on ? sm[s] : 0.0;
+        px[s] = py[s] = pz[s] = 0.0;
+    }
+    for (int i = blockIdx.x * blockDim.x + tid; i < n; i += blockDim.x * gridDim.x) {
+        const double bx = x[i], by = y[i], bz = z[i], bm = mass[i];
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+        for (int s = 0; s < NBX_MAX_STARS; ++s) {
+            if (s >= ns) break;
+            const double dx = qx[s] - bx, dy = qy[s] - by, dz = qz[s] - bz;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 == 0.0) continue;
+            const double r2 = d2 + eps2;
+            const double w = 1.0 / (r2 * sqrt(r2));
+            gx += qm[s] * w * dx; gy += qm[s] * w * dy; gz += qm[s] * w * dz;
+            px[s] -= bm * w * dx; py[s] -= bm * w * dy; pz[s] -= bm * w * dz;
+        }
+        ax[i] += (float)(G * gx);
+        ay[i] += (float)(G * gy);
+        az[i] += (float)(G * gz);
+    }
+#pragma unroll
+    for (int s = 0; s < NBX_MAX_STARS; ++s) {
+        if (s >= ns) break;
+        const double a = wave_sum_f64(px[s]), b = wave_sum_f64(py[s]), c = wave_sum_f64(pz[s]);
+        if (lane == 0) { s_red[wave][s][0] = a; s_red[wave][s][1] = b; s_red[wave][s][2] = c; }
+    }
+    __syncthreads();
+    for (int k = tid; k < 3 * ns; k += blockDim.x) {
+        const int s = k / 3, c = k - 3 * s;
+        double v = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += s_red[w][s][c];
+        part[((int64_t)blockIdx.x * NBX_MAX_STARS + s) * 3 + c] = v;
+    }
+}
+
+__global__ void __launch_bounds__(64) star_final_kernel(
+    const float *__restrict__ sx, const float *__restrict__ sy, const float *__restrict__ sz,
+    const float *__restrict__ sm, int ns, float *__restrict__ sax, float *__restrict__ say, float *__restrict__ saz,
+    double G, double eps2, const double *__restrict__ part, int nblocks, const DeviceState *st)
+{
+    if (latched(st)) return;
+    const int s = threadIdx.x;
+    if (s >= ns) return;
+    const double qx = sx[s], qy = sy[s], qz = sz[s];
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    for (int t = 0; t < ns; ++t) {  // Interaction.cpp:113-118: every other star
+        if (t == s) continue;
+        const double dx = (double)sx[t] - qx, dy = (double)sy[t] - qy, dz = (double)sz[t] - qz;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 == 0.0) continue;
+        const double r2 = d2 + eps2;
+        const double w = (double)sm[t] / (r2 * sqrt(r2));
+        gx += w * dx; gy += w * dy; gz += w * dz;
+    }
+    for (int b = 0; b < nblocks; ++b) {  // :120-123: the bodies, block partials in order
+        const double *p = part + ((int64_t)b * NBX_MAX_STARS + s) * 3;
+        gx += p[0]; gy += p[1]; gz += p[2];
+    }
+    sax[s] = (float)(G * gx);
+    say[s] = (float)(G * gy);
+    saz[s] = (float)(G * gz);
 }
 
 __global__ void fill_f32_kernel(float *p, float v, int64_t n)
@@ -205,17 +306,29 @@ void launch_keys(nbx_context *c, hipStream_t s, const float *x, const float *y, 
     keys_kernel<<<grid_for(n, block, 65536), block, 0, s>>>(x, y, z, mass, n, c->st, key_type, keys, c->bodies4);
 }
 
-void launch_update(hipStream_t s, int block, float *x, float *y, float *z, float *vx, float *vy, float *vz,
-                   const float *ax, const float *ay, const float *az, int n, float dt, float d)
+void launch_update(hipStream_t s, int block, const DeviceState *st, float *x, float *y, float *z, float *vx, float *vy,
+                   float *vz, const float *ax, const float *ay, const float *az, int n, float dt, float d)
 {
     if (block <= 0) block = 256;
-    update_kernel<<<grid_for(n, block, 65536), block, 0, s>>>(x, y, z, vx, vy, vz, ax, ay, az, n, dt, d);
+    update_kernel<<<grid_for(n, block, 65536), block, 0, s>>>(x, y, z, vx, vy, vz, ax, ay, az, n, dt, d, st);
 }
 
 void launch_fill_f32(hipStream_t s, float *p, float v, int64_t n)
 {
     if (n <= 0) return;
     fill_f32_kernel<<<grid_for(n, 256, 4096), 256, 0, s>>>(p, v, n);
+}
+
+void launch_star_forces(nbx_context *c, hipStream_t s, const float *sx, const float *sy, const float *sz,
+                        const float *sm, float *sax, float *say, float *saz, int ns, const float *x, const float *y,
+                        const float *z, const float *mass, float *ax, float *ay, float *az, int n, float g)
+{
+    const double eps2 = (double)layout_eps2(c->p);
+    const int blocks = n > 0 ? grid_for(n, kStarThreads, kStarPartBlocks) : 1;
+    star_bodies_kernel<<<blocks, kStarThreads, 0, s>>>(sx, sy, sz, sm, ns, x, y, z, mass, ax, ay, az, n, (double)g,
+                                                       eps2, c->star_part, c->st);
+    star_final_kernel<<<1, 64, 0, s>>>(sx, sy, sz, sm, ns, sax, say, saz, (double)g, eps2, c->star_part, blocks,
+                                       c->st);
 }
 
 // ------------------------------------------------------------ batched form
@@ -233,6 +346,8 @@ __global__ void __launch_bounds__(256) reset_batch_kernel(int S, int N, int *__r
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *index = N + S;
+        st->sticky |= st->err & ~st->seen;
+        st->seen = 0;
         st->err = 0;
     }
 }

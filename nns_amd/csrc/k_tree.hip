@@ -266,11 +266,15 @@ __global__ void __launch_bounds__(256) subtree_insert_kernel(
 {
     __shared__ int s_job, s_base, s_next, s_bad;
     __shared__ int red[4];
-    if (latched(st)) return;
     const int tid = threadIdx.x;
     const int nsub = __hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
-        if (tid == 0) s_job = atomicAdd(&st->work, 1);
+        // Thread 0 alone reads the error latch (which other workgroups of this
+        // launch may raise at any time) and takes the job; every wave reads its
+        // decision from LDS after the barrier, so the workgroup continues or
+        // leaves as a whole.  A wave that left on its own read would let the
+        // others pass the barrier and read an s_job nobody wrote.
+        if (tid == 0) s_job = latched(st) ? nsub : atomicAdd(&st->work, 1);
         __syncthreads();
         const int job = s_job;
         __syncthreads();
@@ -707,11 +711,13 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
     __shared__ int s_job;
     __shared__ int s_lev[kSumLdsCells];
     __shared__ int s_slot[8 * kSumLdsCells];
-    if (latched(st)) return;
     const int tid = threadIdx.x;
     const int nsub = __hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
-        if (tid == 0) s_job = atomicAdd(&st->work2, 1);
+        // One read of the latch per job, by thread 0, published with the job:
+        // the workgroup continues or leaves as a whole (a zero cell mass raises
+        // the latch from inside this launch).
+        if (tid == 0) s_job = latched(st) ? nsub : atomicAdd(&st->work2, 1);
         __syncthreads();
         const int job = s_job;
         __syncthreads();
@@ -739,9 +745,11 @@ __global__ void __launch_bounds__(1024) top_summary_kernel(SumOut o, Segs g, int
                                                            const int32_t *__restrict__ top_list, DeviceState *st)
 {
     __shared__ int s_cnt[kLevels];
-    if (latched(st)) return;
+    __shared__ int s_stop;
+    if (threadIdx.x == 0) s_stop = latched(st);  // one read for the whole block
     if (threadIdx.x < kLevels) s_cnt[threadIdx.x] = st->topl[threadIdx.x];  // every level's count, read once
     __syncthreads();
+    if (s_stop) return;
     for (int L = kKeyLevels; L >= 1; --L) {
         const int cnt = s_cnt[L];
         if (cnt == 0) continue;  // uniform: no barrier for an empty level

@@ -7,8 +7,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <vector>
 #include <new>
 #include <string>
 
@@ -42,7 +45,8 @@ bool params_ok(const nbx_params *p)
 {
     return p && p->theta >= 0.f && p->softening >= 0.f && p->timestep == p->timestep &&
            p->gravity == p->gravity && p->dampening == p->dampening &&
-           (p->block_size == 0 || (p->block_size >= 64 && p->block_size <= 1024 && p->block_size % 64 == 0));
+           (p->block_size == 0 || (p->block_size >= 64 && p->block_size <= 1024 && p->block_size % 64 == 0)) &&
+           (p->force_waves == 0 || p->force_waves == 2 || p->force_waves == 4 || p->force_waves == 8);
 }
 
 struct StageTimer {
@@ -81,6 +85,8 @@ int check_sizes(nbx_context *c, int n, int m)
 
 }  // namespace
 
+void nbx::set_create_error(const std::string &msg) { g_create_error = msg; }
+
 extern "C" {
 
 void nbx_default_params(nbx_params *p)
@@ -108,6 +114,8 @@ int nbx_create(const nbx_params *p, int n_max, int m_max, int device, nbx_contex
     nbx_context *c = new (std::nothrow) nbx_context();
     if (!c) return fail(nullptr, NBX_ERR_INVALID, "out of host memory");
     c->p = *p;
+    c->force_waves = p->force_waves ? p->force_waves : 4;
+    if (const char *v = std::getenv("NBX_FORCE_RING")) c->force_ring = std::atoi(v) != 0;
     c->device = device;
     c->n_max = n_max;
     c->m_max = m_max;
@@ -130,6 +138,8 @@ int nbx_create(const nbx_params *p, int n_max, int m_max, int device, nbx_contex
               dalloc(&c->froot4, 1) == hipSuccess && dalloc(&c->frootm, 1) == hipSuccess &&
               dalloc(&c->pblk, m) == hipSuccess && dalloc(&c->bpar, n) == hipSuccess && dalloc(&c->counts, n) == hipSuccess &&
               dalloc(&c->acc_scratch, 3 * n) == hipSuccess && dalloc(&c->bodies4, n) == hipSuccess &&
+              dalloc(&c->star_part, (size_t)nbx::kStarPartBlocks * NBX_MAX_STARS * 3) == hipSuccess &&
+              dalloc(&c->walk, 2 * ((n + 63) / 64)) == hipSuccess &&
               dalloc(&c->gkey, n) == hipSuccess && dalloc(&c->gkey2, n) == hipSuccess &&
               dalloc(&c->gorder, n) == hipSuccess &&
               hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
@@ -152,7 +162,7 @@ int nbx_destroy(nbx_context *c)
     hipDeviceSynchronize();
     void *bufs[] = {c->st, c->bbox_part, c->keys, c->skeys, c->kbuf, c->perm, c->vbuf, c->pos, c->hist,
                     c->cell_level, c->mass64, c->mom64, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->pblk, c->bpar, c->counts,
-                    c->acc_scratch, c->bodies4, c->gkey, c->gkey2, c->gorder};
+                    c->acc_scratch, c->star_part, c->walk, c->bodies4, c->gkey, c->gkey2, c->gorder};
     for (void *b : bufs)
         if (b) hipFree(b);
     nbx::free_tree_scratch(c->ts);
@@ -170,6 +180,7 @@ int nbx_set_params(nbx_context *c, const nbx_params *p)
     if (!c) return NBX_ERR_INVALID;
     if (!params_ok(p)) return fail(c, NBX_ERR_INVALID, "invalid parameters");
     c->p = *p;
+    if (p->force_waves) c->force_waves = p->force_waves;  // 0 keeps the current (possibly autotuned) choice
     return NBX_OK;
 }
 
@@ -177,6 +188,7 @@ int nbx_get_params(const nbx_context *c, nbx_params *p)
 {
     if (!c || !p) return NBX_ERR_INVALID;
     *p = c->p;
+    p->force_waves = c->force_waves;
     return NBX_OK;
 }
 
@@ -190,8 +202,20 @@ int nbx_check(nbx_context *c, void *stream)
     if (!c) return NBX_ERR_INVALID;
     hipSetDevice(c->device);
     NBX_HIP(c, hipStreamSynchronize((hipStream_t)stream));
-    int err = 0;
-    NBX_HIP(c, hipMemcpy(&err, &c->st->err, sizeof(int), hipMemcpyDeviceToHost));
+    // e[0] err: this step's latch; e[1] sticky: unreported errors of earlier
+    // steps (moved there by the reset stage); e[2] seen.  Reported here once:
+    // sticky is cleared and this step's bits are marked seen, so that the next
+    // reset does not report them again.
+    int e[3] = {0, 0, 0};
+    static_assert(offsetof(DeviceState, sticky) == offsetof(DeviceState, err) + sizeof(int) &&
+                      offsetof(DeviceState, seen) == offsetof(DeviceState, err) + 2 * sizeof(int),
+                  "err, sticky and seen are adjacent");
+    NBX_HIP(c, hipMemcpy(e, &c->st->err, 3 * sizeof(int), hipMemcpyDeviceToHost));
+    if (e[1] || e[2] != e[0]) {
+        const int upd[2] = {0, e[0]};
+        NBX_HIP(c, hipMemcpy(&c->st->sticky, upd, 2 * sizeof(int), hipMemcpyHostToDevice));
+    }
+    const int err = e[0] | e[1];
     if (err & nbx::kErrNonFinite) return fail(c, NBX_ERR_NONFINITE, "non-finite position or zero cell mass");
     if (err & nbx::kErrCapacity) return fail(c, NBX_ERR_CAPACITY, "tree needs more than m - n cells");
     if (err & nbx::kErrBucket) return fail(c, NBX_ERR_BUCKET, "more than 8 bodies share one key");
@@ -232,6 +256,7 @@ int nbx_compute_bounding_box(nbx_context *c, int *mutex, float *x, float *y, flo
     StageTimer t(c, s, elapsed_ms);
     nbx::launch_bbox(c, s, x, y, z, minX, maxX, minY, maxY, minZ, maxZ, n);
     c->stage = 1;
+    c->bbox_x = x; c->bbox_y = y; c->bbox_z = z; c->bbox_n = n;
     return t.finish();
 }
 
@@ -247,6 +272,8 @@ int nbx_build_tree(nbx_context *c, float *x, float *y, float *z, float *mass, in
         return fail(c, NBX_ERR_INVALID, "null array");
     if (c->stage < 1) return fail(c, NBX_ERR_ORDER, "nbx_build_tree needs nbx_compute_bounding_box first");
     if (!c->fresh) return fail(c, NBX_ERR_ORDER, "nbx_build_tree needs nbx_reset_arrays first");
+    if (n != c->bbox_n || x != c->bbox_x || y != c->bbox_y || z != c->bbox_z)
+        return fail(c, NBX_ERR_INVALID, "bodies differ from those of the last nbx_compute_bounding_box");
     c->fresh = false;
     hipSetDevice(c->device);
     hipStream_t s = (hipStream_t)stream;
@@ -319,7 +346,26 @@ int nbx_update(nbx_context *c, float *x, float *y, float *z, float *vx, float *v
     hipSetDevice(c->device);
     hipStream_t s = (hipStream_t)stream;
     StageTimer t(c, s, elapsed_ms);
-    nbx::launch_update(s, c->p.block_size, x, y, z, vx, vy, vz, ax, ay, az, n, dt, d);
+    nbx::launch_update(s, c->p.block_size, c->st, x, y, z, vx, vy, vz, ax, ay, az, n, dt, d);
+    return t.finish();
+}
+
+int nbx_star_forces(nbx_context *c, const float *sx, const float *sy, const float *sz, const float *smass,
+                    float *sax, float *say, float *saz, int ns, const float *x, const float *y, const float *z,
+                    const float *mass, float *ax, float *ay, float *az, int n, float g, void *stream,
+                    float *elapsed_ms)
+{
+    if (!c) return NBX_ERR_INVALID;
+    if (ns < 0 || ns > NBX_MAX_STARS) return fail(c, NBX_ERR_INVALID, "ns out of range [0, NBX_MAX_STARS]");
+    if (n < 0 || n > c->n_max) return fail(c, NBX_ERR_INVALID, "n out of range [0, n_max]");
+    if (ns > 0 && (!sx || !sy || !sz || !smass || !sax || !say || !saz))
+        return fail(c, NBX_ERR_INVALID, "null star array");
+    if (n > 0 && (!x || !y || !z || !mass || !ax || !ay || !az)) return fail(c, NBX_ERR_INVALID, "null array");
+    if (!std::isfinite(g)) return fail(c, NBX_ERR_INVALID, "g must be finite");
+    hipSetDevice(c->device);
+    hipStream_t s = (hipStream_t)stream;
+    StageTimer t(c, s, elapsed_ms);
+    if (ns > 0) nbx::launch_star_forces(c, s, sx, sy, sz, smass, sax, say, saz, ns, x, y, z, mass, ax, ay, az, n, g);
     return t.finish();
 }
 
@@ -332,6 +378,8 @@ int nbx_compute_keys(nbx_context *c, const float *x, const float *y, const float
     if (key_type != NBX_KEY_MORTON && key_type != NBX_KEY_HILBERT)
         return fail(c, NBX_ERR_INVALID, "unknown key type");
     if (c->stage < 1) return fail(c, NBX_ERR_ORDER, "nbx_compute_keys needs nbx_compute_bounding_box first");
+    if (n != c->bbox_n || x != c->bbox_x || y != c->bbox_y || z != c->bbox_z)
+        return fail(c, NBX_ERR_INVALID, "bodies differ from those of the last nbx_compute_bounding_box");
     hipSetDevice(c->device);
     hipStream_t s = (hipStream_t)stream;
     StageTimer t(c, s, elapsed_ms);
@@ -367,6 +415,34 @@ int nbx_count_interactions(nbx_context *c, int n, int64_t *counts, void *stream)
     return t.finish();
 }
 
+int nbx_walk_stats(nbx_context *c, int64_t *out, void *stream)
+{
+    if (!c || !out) return NBX_ERR_INVALID;
+    if (c->stage < 4) return fail(c, NBX_ERR_ORDER, "nbx_walk_stats needs nbx_sort first");
+    if (!c->force_ring) return fail(c, NBX_ERR_INVALID, "walk statistics need the LDS-staged force walk");
+    hipSetDevice(c->device);
+    hipStream_t s = (hipStream_t)stream;
+    const int n = c->n, ng = (n + 63) / 64;
+    float *scratch = c->acc_scratch;
+    nbx::launch_force(c, s, scratch, scratch + n, scratch + 2 * (size_t)n, 1.f, c->counts, c->walk);
+    NBX_HIP(c, hipGetLastError());
+    NBX_HIP(c, hipStreamSynchronize(s));
+    std::vector<int64_t> cnt(n), wk(2 * (size_t)ng);
+    NBX_HIP(c, hipMemcpy(cnt.data(), c->counts, 8 * (size_t)n, hipMemcpyDeviceToHost));
+    NBX_HIP(c, hipMemcpy(wk.data(), c->walk, 16 * (size_t)ng, hipMemcpyDeviceToHost));
+    int64_t inter = 0, steps = 0, frames = 0;
+    for (int64_t v : cnt) inter += v;
+    for (int k = 0; k < ng; ++k) {
+        steps += wk[2 * (size_t)k];
+        frames += wk[2 * (size_t)k + 1];
+    }
+    out[0] = inter;
+    out[1] = steps;
+    out[2] = frames;
+    out[3] = ng;
+    return NBX_OK;
+}
+
 int nbx_force_vjp(nbx_context *c, const float *ux, const float *uy, const float *uz, float g, float *gx, float *gy,
                   float *gz, float *gm, void *stream, float *elapsed_ms)
 {
@@ -387,6 +463,11 @@ int nbx_step(nbx_context *c, const nbx_arrays *a, void *stream, float *stage_ms)
     if (!c || !a) return NBX_ERR_INVALID;
     if (int r = check_sizes(c, a->n, a->m)) return r;
     if (a->n < 1) return fail(c, NBX_ERR_INVALID, "n must be >= 1");
+    const int ns = a->nstars;
+    if (ns < 0 || ns > NBX_MAX_STARS) return fail(c, NBX_ERR_INVALID, "nstars out of range [0, NBX_MAX_STARS]");
+    if (ns > 0 && (!a->sx || !a->sy || !a->sz || !a->smass || !a->svx || !a->svy || !a->svz || !a->sax || !a->say ||
+                   !a->saz))
+        return fail(c, NBX_ERR_INVALID, "null star array");
     hipSetDevice(c->device);
     hipStream_t s = (hipStream_t)stream;
     const bool tm = stage_ms != nullptr;
@@ -410,9 +491,17 @@ int nbx_step(nbx_context *c, const nbx_arrays *a, void *stream, float *stage_ms)
     mark(5);
     if (!r) r = nbx_compute_forces(c, a->x, a->y, a->z, a->vx, a->vy, a->vz, a->ax, a->ay, a->az, a->mass,
                                    a->sorted, a->child, a->minX, a->maxX, a->n, c->p.gravity, stream, nullptr);
+    if (!r && ns > 0)
+        r = nbx_star_forces(c, a->sx, a->sy, a->sz, a->smass, a->sax, a->say, a->saz, ns, a->x, a->y, a->z, a->mass,
+                            a->ax, a->ay, a->az, a->n, c->p.gravity, stream, nullptr);
     mark(6);
     if (!r) r = nbx_update(c, a->x, a->y, a->z, a->vx, a->vy, a->vz, a->ax, a->ay, a->az, a->n, c->p.timestep,
                            c->p.dampening, stream, nullptr);
+    if (!r && ns > 0) {  // the stars move with the bodies (Interaction.cpp:219-234)
+        nbx::launch_update(s, 64, c->st, a->sx, a->sy, a->sz, a->svx, a->svy, a->svz, a->sax, a->say, a->saz, ns,
+                           c->p.timestep, c->p.dampening);
+        if (hipGetLastError() != hipSuccess) r = fail(c, NBX_ERR_HIP, "kernel launch: star update");
+    }
     mark(7);
     if (r) return r;
     if (tm) {

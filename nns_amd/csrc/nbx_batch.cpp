@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -54,7 +56,7 @@ void enqueue_step(nbx_batch *b, hipStream_t s, hipEvent_t *ev)
     mark(5);
     nbx::launch_batch_force(b, s, nullptr, b->force_waves);
     mark(6);
-    nbx::launch_update(s, b->p.block_size, b->x, b->y, b->z, b->vx, b->vy, b->vz, b->ax, b->ay, b->az, b->N,
+    nbx::launch_update(s, b->p.block_size, b->st, b->x, b->y, b->z, b->vx, b->vy, b->vz, b->ax, b->ay, b->az, b->N,
                        b->p.timestep, b->p.dampening);
     mark(7);
 }
@@ -80,6 +82,8 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
     *out = nullptr;
     if (!p || nsys < 1 || !sys_n) return bfail(nullptr, NBX_ERR_INVALID, "need params, nsys >= 1 and sys_n");
     if (nsys > 65535) return bfail(nullptr, NBX_ERR_INVALID, "at most 65535 systems");
+    if (p->force_waves != 0 && p->force_waves != 2 && p->force_waves != 4 && p->force_waves != 8)
+        return bfail(nullptr, NBX_ERR_INVALID, "force_waves must be 0, 2, 4 or 8");
     int64_t total = 0, samples = 0;
     for (int s = 0; s < nsys; ++s) {
         if (sys_n[s] < 1) return bfail(nullptr, NBX_ERR_INVALID, "every system needs >= 1 body");
@@ -102,6 +106,7 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
     b->N = (int)total;
     b->m = (int)m64;
     b->force_waves = p->force_waves == 2 || p->force_waves == 8 ? p->force_waves : 4;
+    if (const char *v = std::getenv("NBX_FORCE_RING")) b->force_ring = std::atoi(v) != 0;
     std::vector<int32_t> off(nsys + 1, 0), sys_of(total), samp(nsys + 1, 0);
     std::vector<float> th(nsys, p->theta);
     std::vector<int4> groups;
@@ -125,6 +130,7 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
               dalloc(&b->mass64, M) && dalloc(&b->mom64, 3 * M) && dalloc(&b->nodes4, M) &&
               dalloc(&b->fnode, 8 * M) && dalloc(&b->fmeta, 8 * M) && dalloc(&b->froot4, S) &&
               dalloc(&b->frootm, S) && dalloc(&b->pblk, M) && dalloc(&b->bpar, N) && dalloc(&b->counts, N) && dalloc(&b->acc_scratch, 3 * N) && dalloc(&b->bodies4, N) &&
+              dalloc(&b->walk, 2 * groups.size()) &&
               dalloc(&b->gkey, N) && dalloc(&b->gkey2, N) && dalloc(&b->gorder, N) &&
               dalloc(&b->samp_off, S + 1) && nbx::alloc_tree_scratch(b->ts, (int)N, nsys, samp[nsys]);
     for (int k = 0; ok && k < 8; ++k) ok = hipEventCreate(&b->stage_ev[k]) == hipSuccess;
@@ -152,7 +158,7 @@ int nbx_batch_destroy(nbx_batch *b)
     void *bufs[] = {b->st, b->off, b->sys_of, b->theta, b->groups, b->ext, b->mm, b->keys, b->skeys, b->kbuf,
                     b->k2, b->k2buf, b->perm, b->perm1, b->vbuf, b->pos, b->sorted, b->hist, b->child, b->index,
                     b->count, b->start, b->cell_level, b->mass64, b->mom64, b->nodes4,
-                    b->fnode, b->fmeta, b->froot4, b->frootm, b->pblk, b->bpar, b->counts, b->acc_scratch, b->samp_off, b->bodies4, b->gkey, b->gkey2, b->gorder};
+                    b->fnode, b->fmeta, b->froot4, b->frootm, b->pblk, b->bpar, b->counts, b->acc_scratch, b->samp_off, b->bodies4, b->gkey, b->gkey2, b->gorder, b->walk};
     for (void *q : bufs)
         if (q) hipFree(q);
     nbx::free_tree_scratch(b->ts);
@@ -221,9 +227,20 @@ int nbx_batch_check(nbx_batch *b, void *stream)
     if (!b) return NBX_ERR_INVALID;
     hipSetDevice(b->device);
     if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return bfail(b, NBX_ERR_HIP, "hipStreamSynchronize");
-    int err = 0;
-    if (hipMemcpy(&err, &b->st->err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    // err, sticky and seen as in nbx_check: reported once, then sticky cleared
+    // and this step's bits marked seen
+    int e[3] = {0, 0, 0};
+    static_assert(offsetof(nbx::DeviceState, sticky) == offsetof(nbx::DeviceState, err) + sizeof(int) &&
+                      offsetof(nbx::DeviceState, seen) == offsetof(nbx::DeviceState, err) + 2 * sizeof(int),
+                  "err, sticky and seen are adjacent");
+    if (hipMemcpy(e, &b->st->err, 3 * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
         return bfail(b, NBX_ERR_HIP, "hipMemcpy");
+    if (e[1] || e[2] != e[0]) {
+        const int upd[2] = {0, e[0]};
+        if (hipMemcpy(&b->st->sticky, upd, 2 * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+            return bfail(b, NBX_ERR_HIP, "hipMemcpy");
+    }
+    const int err = e[0] | e[1];
     if (err & nbx::kErrNonFinite) return bfail(b, NBX_ERR_NONFINITE, "non-finite position or zero cell mass");
     if (err & nbx::kErrCapacity) return bfail(b, NBX_ERR_CAPACITY, "tree needs more cells than the ensemble holds");
     if (err & nbx::kErrBucket) return bfail(b, NBX_ERR_BUCKET, "more than 8 bodies share one key");
@@ -237,6 +254,32 @@ int nbx_batch_count_interactions(nbx_batch *b, int64_t *counts, void *stream)
     hipSetDevice(b->device);
     nbx::launch_batch_force(b, (hipStream_t)stream, counts, b->force_waves);
     return finish(b);
+}
+
+int nbx_batch_walk_stats(nbx_batch *b, int64_t *out, void *stream)
+{
+    if (!b || !out) return NBX_ERR_INVALID;
+    if (!b->force_ring) return bfail(b, NBX_ERR_INVALID, "walk statistics need the LDS-staged force walk");
+    hipSetDevice(b->device);
+    hipStream_t s = (hipStream_t)stream;
+    nbx::launch_batch_force(b, s, b->counts, b->force_waves, b->walk);
+    if (int r = finish(b)) return r;
+    if (hipStreamSynchronize(s) != hipSuccess) return bfail(b, NBX_ERR_HIP, "hipStreamSynchronize");
+    std::vector<int64_t> cnt(b->N), wk(2 * (size_t)b->ngroups);
+    if (hipMemcpy(cnt.data(), b->counts, 8 * (size_t)b->N, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(wk.data(), b->walk, 16 * (size_t)b->ngroups, hipMemcpyDeviceToHost) != hipSuccess)
+        return bfail(b, NBX_ERR_HIP, "copy failed");
+    int64_t inter = 0, steps = 0, frames = 0;
+    for (int64_t v : cnt) inter += v;
+    for (int k = 0; k < b->ngroups; ++k) {
+        steps += wk[2 * (size_t)k];
+        frames += wk[2 * (size_t)k + 1];
+    }
+    out[0] = inter;
+    out[1] = steps;
+    out[2] = frames;
+    out[3] = b->ngroups;
+    return NBX_OK;
 }
 
 int nbx_batch_autotune(nbx_batch *b, void *stream, int *waves_out)

@@ -8,12 +8,17 @@ namespace nbx {
 
 // ---------------------------------------------------------------- keys
 // i = (uint)(MASK * (p - o) / maxSpan) in FP64, truncated
-// (3D/OpenMP/OpenMP_NBody/src/Body.cpp:14-16).  maxSpan == 0 maps to 0.
+// (3D/OpenMP/OpenMP_NBody/src/Body.cpp:14-16).  maxSpan == 0 maps to 0.  A body
+// inside the extent gives t in [0, MASK]; t is clamped to that range (NaN to 0)
+// before the cast, so a body outside it cannot make a negative or over-range
+// coordinate (whose conversion is undefined) -- values inside are unchanged.
 static __device__ inline uint64_t quantise(float p, double o, double s)
 {
     if (!(s > 0.0)) return 0;
     double t = kMaskCoord * ((double)p - o);
     t = t / s;
+    t = t >= 0.0 ? t : 0.0;
+    t = t <= kMaskCoord ? t : kMaskCoord;
     return (uint64_t)t;
 }
 

@@ -52,7 +52,9 @@ struct DeviceState {
     double ext[4];      // xMin, yMin, zMin, maxSpan
     double root64[4];   // root mass, com x, y, z
     float4 root4;       // root com x, y, z, mass (float)
-    int err;            // latched kErr* bits
+    int err;            // latched kErr* bits of the current step (cleared by the reset stage)
+    int sticky;         // unreported bits of earlier steps, moved here by the reset stage
+    int seen;           // bits of err already reported by the host check (not moved to sticky)
     unsigned bbox_ticket;
     int nsub;           // subtrees found by the top-tree pass
     int work;           // next subtree to build
@@ -74,6 +76,9 @@ struct TreeScratch {
     uint32_t *scan_tmp = nullptr;  // 1024 block sums of the top-table scan
     int samples_max = 0, sub_max = 0;
 };
+
+// Message returned by nbx_last_error(NULL) (errors of calls without a context).
+void set_create_error(const std::string &msg);
 
 // Allocates the scratch for up to n bodies in nseg segments with `samples` top-table
 // samples in total; returns false on allocation failure.
@@ -118,6 +123,7 @@ struct nbx_context {
     float layout_eps2 = -1.f;    // and eps^2
     int64_t *counts = nullptr;
     float *acc_scratch = nullptr;
+    double *star_part = nullptr;  // star forces: per-block partial pulls, kStarBlocks x NBX_MAX_STARS x 3
     float4 *bodies4 = nullptr;  // (x, y, z, m) of every body in input order
     uint32_t *gkey = nullptr, *gkey2 = nullptr;  // group-order sort keys
     int32_t *gorder = nullptr;  // [n] sorted position of the j-th body of the force groups
@@ -130,8 +136,13 @@ struct nbx_context {
     int *child = nullptr, *count = nullptr, *start = nullptr, *index = nullptr;
     int n = 0, m = 0;
     int stage = 0;  // 0 reset, 1 bbox, 2 built, 3 summarised, 4 sorted, 5 forces
+    // bodies the FP64 extent of the last nbx_compute_bounding_box was taken over
+    const float *bbox_x = nullptr, *bbox_y = nullptr, *bbox_z = nullptr;
+    int bbox_n = -1;
     bool fresh = false;  // arrays reset since the last build (root slots are -1)
     int force_waves = 4; // waves per block of the force kernel (2, 4 or 8; autotuned)
+    bool force_ring = false;     // LDS-staged force walk
+    int64_t *walk = nullptr;     // [2 ceil(n_max / 64)] child steps and frames per force group
 };
 
 // Ensemble of S independent systems (batched form).  System s owns bodies
@@ -145,6 +156,8 @@ struct nbx_batch {
     int ngroups = 0;
     std::string last_error;
     int force_waves = 4;
+    bool force_ring = false;
+    int64_t *walk = nullptr;  // [2 ngroups]
     // device
     nbx::DeviceState *st = nullptr;
     int32_t *off = nullptr, *sys_of = nullptr;
@@ -191,9 +204,15 @@ void launch_bbox(nbx_context *c, hipStream_t s, const float *x, const float *y, 
 // Also packs (x, y, z, mass) into c->bodies4 when mass is given.
 void launch_keys(nbx_context *c, hipStream_t s, const float *x, const float *y, const float *z, const float *mass,
                  int n, int key_type, uint64_t *keys);
-void launch_update(hipStream_t s, int block, float *x, float *y, float *z, float *vx, float *vy, float *vz,
-                   const float *ax, const float *ay, const float *az, int n, float dt, float d);
+// Does nothing once the error latch of st is raised (st may be null: always runs).
+void launch_update(hipStream_t s, int block, const DeviceState *st, float *x, float *y, float *z, float *vx, float *vy,
+                   float *vz, const float *ax, const float *ay, const float *az, int n, float dt, float d);
 void launch_fill_f32(hipStream_t s, float *p, float v, int64_t n);
+// Stars outside the tree (nbx_star_forces); scratch c->star_part.
+constexpr int kStarPartBlocks = 1024;
+void launch_star_forces(nbx_context *c, hipStream_t s, const float *sx, const float *sy, const float *sz,
+                        const float *sm, float *sax, float *say, float *saz, int ns, const float *x, const float *y,
+                        const float *z, const float *mass, float *ax, float *ay, float *az, int n, float g);
 
 // launchers (k_radix.hip)
 // Sorts keys_in (not modified) by their low key_bits bits into keys_out / perm_out (stable);
@@ -230,7 +249,10 @@ inline bool frame_layout_stale(const nbx_context *c)
 constexpr int kGroupLevels = 10;
 
 // launchers (k_force.hip)
-void launch_force(nbx_context *c, hipStream_t s, float *ax, float *ay, float *az, float g, int64_t *counts);
+// counts (nullable): per-body interaction counts; walk (nullable, with counts): per
+// group, the child steps and frames of its walk.
+void launch_force(nbx_context *c, hipStream_t s, float *ax, float *ay, float *az, float g, int64_t *counts,
+                  int64_t *walk = nullptr);
 void launch_force_waves(nbx_context *c, hipStream_t s, float *ax, float *ay, float *az, float g, int waves);
 
 // batched form (k_basic.hip, k_radix.hip, k_tree.hip, k_force.hip)
@@ -241,7 +263,7 @@ void launch_batch_sort(nbx_batch *b, hipStream_t s);
 void launch_batch_build(nbx_batch *b, hipStream_t s);
 void launch_batch_summarize(nbx_batch *b, hipStream_t s);
 void launch_batch_layout(nbx_batch *b, hipStream_t s);
-void launch_batch_force(nbx_batch *b, hipStream_t s, int64_t *counts, int waves);
+void launch_batch_force(nbx_batch *b, hipStream_t s, int64_t *counts, int waves, int64_t *walk = nullptr);
 
 // launchers (k_adjoint.hip)
 void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy, const float *uz, float *gx,

@@ -31,6 +31,9 @@
  *                        singleInteraction (Interaction.cpp:17-55).
  *   orc_direct_accel     FP64 direct sum, the closed form a_i = sum_j m_j d_ij (|d_ij|^2+eps^2)^-3/2.
  *   orc_direct_accel_fast the same sum vectorised over sources (summation order differs).
+ *   orc_star_forces      the reference's suns: bodies outside the tree, direct and
+ *                        symmetric (Interaction.cpp:111-124), SI or Plummer G = 1 form
+ *   orc_update_reference Interaction::updateBodies (Interaction.cpp:219-255)
  *   orc_update           symplectic Euler of updateKernel (Kernels.cu:581-600), written with
  *                        explicit fused multiply-adds so it is bit-reproducible.
  *
@@ -820,5 +823,67 @@ void orc_geo_accel(const orc_geo *g, int64_t n, const double *x, const double *y
         if (geo_contains(&g->nodes[0], x[i], y[i], z[i])) geo_walk(g, 0, &b, theta, &u, a, &cnt);
         acc[3 * i] = a[0]; acc[3 * i + 1] = a[1]; acc[3 * i + 2] = a[2];
         if (ninter) ninter[i] = cnt;
+    }
+}
+
+/* ============================================ stars ("suns") outside the tree */
+/* The star part of Interaction::interactBodies (Interaction.cpp:111-124) in the
+ * order one thread runs it: for each sun s, every other sun ss
+ * (singleInteraction(&suns[s], &suns[ss], false): only s is accelerated), then
+ * every body b (singleInteraction(&suns[s], &bods[b], true): the one force of
+ * the pair accelerates both).  Suns are not in the tree.
+ * kernel 0: the reference's SI form (velocity increments, the expressions of
+ * Interaction.cpp:17-55); kernel 1: Plummer with G = 1 (accelerations, the
+ * engine's form: sun s gets sum m_b (x_b - x_s) r^-3, body b gets
+ * sum m_s (x_s - x_b) r^-3, zero distances left out).
+ * sun_acc (3 ns) and body_acc (3 nb) are added to. */
+void orc_star_forces(int64_t ns, const double *sm, const double *sx, const double *sy, const double *sz,
+                     int64_t nb, const double *bm, const double *bx, const double *by, const double *bz,
+                     int kernel, double to_meters, double grav, double time_step, double softening,
+                     double eps2, double *sun_acc, double *body_acc)
+{
+    geo_units u = {to_meters, grav, time_step, softening, eps2, kernel};
+    for (int64_t s = 0; s < ns; ++s) {
+        const gbody a = {sm[s], sx[s], sy[s], sz[s]};
+        double *as = sun_acc + 3 * s;
+        for (int64_t ss = 0; ss < ns; ++ss) {
+            if (ss == s) continue;
+            const gbody b = {sm[ss], sx[ss], sy[ss], sz[ss]};
+            geo_single(&u, &a, &b, as);
+        }
+        for (int64_t i = 0; i < nb; ++i) {
+            const gbody b = {bm[i], bx[i], by[i], bz[i]};
+            double *ab = body_acc + 3 * i;
+            if (kernel == 0) {
+                const double pdx = (a.x - b.x) * u.tm, pdy = (a.y - b.y) * u.tm, pdz = (a.z - b.z) * u.tm;
+                const double distance = sqrt(pdx * pdx + pdy * pdy + pdz * pdz);
+                if (distance == 0) continue;
+                const double force = u.ts * (u.g * a.mass * b.mass) / ((distance * distance + u.soft * u.soft) * distance);
+                as[0] -= force * pdx / a.mass;
+                as[1] -= force * pdy / a.mass;
+                as[2] -= force * pdz / a.mass;
+                ab[0] += force * pdx / b.mass;
+                ab[1] += force * pdy / b.mass;
+                ab[2] += force * pdz / b.mass;
+            } else {
+                geo_single(&u, &a, &b, as);
+                geo_single(&u, &b, &a, ab);
+            }
+        }
+    }
+}
+
+/* Interaction::updateBodies (Interaction.cpp:219-255) for one set of bodies:
+ * v += a (the velocity increment of the step), a = 0,
+ * x += TIME_STEP * v / TO_METERS. */
+void orc_update_reference(int64_t n, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                          double *acc, double time_step, double to_meters)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        vx[i] += acc[3 * i]; vy[i] += acc[3 * i + 1]; vz[i] += acc[3 * i + 2];
+        acc[3 * i] = 0.0; acc[3 * i + 1] = 0.0; acc[3 * i + 2] = 0.0;
+        x[i] += time_step * vx[i] / to_meters;
+        y[i] += time_step * vy[i] / to_meters;
+        z[i] += time_step * vz[i] / to_meters;
     }
 }

@@ -86,6 +86,9 @@ def lib():
                                 C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                 C.c_double, _f64p, C.c_void_p]
     L.orc_geo_free.argtypes = [C.c_void_p]
+    L.orc_star_forces.argtypes = [C.c_int64] + [_f64p] * 4 + [C.c_int64] + [_f64p] * 4 + \
+        [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, _f64p, _f64p]
+    L.orc_update_reference.argtypes = [C.c_int64] + [_f64p] * 7 + [C.c_double, C.c_double]
     _lib = L
     return L
 
@@ -308,3 +311,51 @@ def ref_tree(x, y, z, m, theta, softening_m, center, length):
     rec = np.dtype([("level", "<i4"), ("external", "<i4"), ("d", "<f8", (8,))])
     nodes = np.frombuffer(buf, rec, nn, off).copy()
     return dv, nodes
+
+
+# ------------------------------------------------------------ stars ("suns")
+def star_forces(suns, bodies, kernel=1, eps2=0.0, softening_m=0.0, time_step=1.0):
+    """Direct, symmetric forces between the suns and the bodies (suns are not in
+    the tree; Interaction.cpp:111-124).  suns, bodies: (m, x, y, z) arrays.
+    kernel 1: Plummer G = 1 accelerations; kernel 0: the reference's SI
+    velocity increments.  Returns (sun_acc [ns, 3], body_acc [nb, 3])."""
+    sm, sx, sy, sz = (_f64(a) for a in suns)
+    bm, bx, by, bz = (_f64(a) for a in bodies)
+    sa = np.zeros(3 * len(sm))
+    ba = np.zeros(3 * len(bm))
+    lib().orc_star_forces(len(sm), sm, sx, sy, sz, len(bm), bm, bx, by, bz, int(kernel), TO_METERS, G_SI,
+                          float(time_step), float(softening_m), float(eps2), sa, ba)
+    return sa.reshape(-1, 3), ba.reshape(-1, 3)
+
+
+def update_reference(x, y, z, vx, vy, vz, dv, time_step):
+    """Interaction::updateBodies (Interaction.cpp:219-255): v += dv; x += TIME_STEP v / TO_METERS."""
+    a = [_f64(v).copy() for v in (x, y, z, vx, vy, vz)]
+    acc = _f64(dv).reshape(-1).copy()
+    lib().orc_update_reference(len(a[0]), *a, acc, float(time_step), TO_METERS)
+    return a
+
+
+def ref_stars(suns, bodies, softening_m, time_step):
+    """The reference's star loop and update (needs oracle/_ref).  suns, bodies:
+    (m, x, y, z, vx, vy, vz) arrays.  Returns dict of sun_dv, body_dv, and the
+    positions and velocities after Interaction::updateBodies."""
+    s = [_f64(a) for a in suns]
+    b = [_f64(a) for a in bodies]
+    ns, nb = len(s[0]), len(b[0])
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<qq", ns, nb))
+            for a in s + b:
+                f.write(a.tobytes())
+        subprocess.run([REF_HARNESS, "stars", fin, fout, repr(float(softening_m)), str(int(time_step))], check=True)
+        buf = open(fout, "rb").read()
+    ns2, nb2 = struct.unpack_from("<qq", buf, 0)
+    off = 16
+    out = {}
+    for key, k in (("sun_dv", ns2), ("body_dv", nb2), ("sun_x", ns2), ("sun_v", ns2), ("body_x", nb2),
+                   ("body_v", nb2)):
+        out[key] = np.frombuffer(buf, np.float64, 3 * k, off).reshape(k, 3).copy()
+        off += 24 * k
+    return out

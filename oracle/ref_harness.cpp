@@ -26,16 +26,26 @@
 //       order (Interaction.cpp:179-186) and Interaction::treeInteraction per body
 //       (:203-208) with TIME_STEP = 1 and SOFTENING = softening_m.  dv is the
 //       velocity increment the reference accumulates in Body::acceleration.
-//   time IN theta softening_m nsample [hashed]
+//   time IN theta softening_m nsample [hashed [reps]]
 //       IN : int64 n, float x[n], y[n], z[n], m[n]
 //       Times the serial Tree::insert of all n bodies and the OpenMP
-//       treeInteraction loop over every (n/nsample)-th body; prints one JSON line.
+//       treeInteraction loop over every (n/nsample)-th body, reps times (a new
+//       tree each time); prints one JSON line with the medians and every rep.
 //       hashed = 1 walks the bodies in key order, the reference's `hashed` path
 //       (Interaction.cpp:156-177,190-197): Body::discretizePosition +
 //       Body::getKey per body into a std::map<key, Body*> (a repeated key keeps
 //       the last body), and the sample is every (n/nsample)-th body of that
 //       order.  The map is timed on its own (key_s), as the reference leaves it
 //       outside its build and force timers.
+//   stars IN OUT softening_m time_step
+//       IN : int64 ns, int64 nb, then for the ns suns and the nb bodies in turn
+//            double m[], x[], y[], z[], vx[], vy[], vz[]
+//       OUT: int64 ns, nb; the velocity increments of the star loop of
+//            Interaction::interactBodies (Interaction.cpp:111-124) for the suns
+//            (3 ns doubles) and bodies (3 nb); then, after
+//            Interaction::updateBodies (:219-255), sun positions and
+//            velocities, body positions and velocities (3 doubles each).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -178,7 +188,7 @@ int cmd_tree(const char *in, const char *out, double theta, double soft, double 
     return 0;
 }
 
-int cmd_time(const char *in, double theta, double soft, int64_t nsample, int hashed)
+int cmd_time(const char *in, double theta, double soft, int64_t nsample, int hashed, int reps)
 {
     FILE *f = fopen(in, "rb");
     if (!f) return 2;
@@ -218,31 +228,113 @@ int cmd_time(const char *in, double theta, double soft, int64_t nsample, int has
         for (int64_t i = 0; i < n; ++i) order.push_back(&bods[i]);
     }
     const int64_t norder = (int64_t)order.size();
-    auto t0 = std::chrono::steady_clock::now();
-    Tree *tree = new Tree(Octant(cx, cy, cz, len));
-    int64_t inserted = 0;
-    for (int64_t i = 0; i < n; ++i)
-        if (tree->getOctant().contains(bods[i].position)) { tree->insert(&bods[i]); ++inserted; }
-    auto t1 = std::chrono::steady_clock::now();
     if (nsample <= 0 || nsample > n) nsample = n;
     int64_t stride = n / nsample;
     if (stride < 1) stride = 1;
-    int64_t ns = (norder + stride - 1) / stride;
-    #pragma omp parallel for
-    for (int64_t k = 0; k < ns; ++k) {
-        Body *b = order[k * stride];
-        if (tree->getOctant().contains(b->position)) inter.treeInteraction(tree, b);
+    const int64_t ns = (norder + stride - 1) / stride;
+    if (reps < 1) reps = 1;
+    std::vector<double> build_s(reps), force_s(reps);
+    int64_t inserted = 0;
+    for (int r = 0; r < reps; ++r) {  // one step each: a fresh tree, then the sampled walks
+        auto t0 = std::chrono::steady_clock::now();
+        Tree *tree = new Tree(Octant(cx, cy, cz, len));
+        inserted = 0;
+        for (int64_t i = 0; i < n; ++i)
+            if (tree->getOctant().contains(bods[i].position)) { tree->insert(&bods[i]); ++inserted; }
+        auto t1 = std::chrono::steady_clock::now();
+        #pragma omp parallel for
+        for (int64_t k = 0; k < ns; ++k) {
+            Body *b = order[k * stride];
+            if (tree->getOctant().contains(b->position)) inter.treeInteraction(tree, b);
+        }
+        auto t2 = std::chrono::steady_clock::now();
+        build_s[r] = std::chrono::duration<double>(t1 - t0).count();
+        force_s[r] = std::chrono::duration<double>(t2 - t1).count();
+        delete tree;
     }
-    auto t2 = std::chrono::steady_clock::now();
-    double build_s = std::chrono::duration<double>(t1 - t0).count();
-    double force_s = std::chrono::duration<double>(t2 - t1).count();
+    auto median = [](std::vector<double> v) {
+        std::sort(v.begin(), v.end());
+        const size_t h = v.size() / 2;
+        return v.size() % 2 ? v[h] : 0.5 * (v[h - 1] + v[h]);
+    };
+    auto list = [](const std::vector<double> &v) {
+        std::string s = "[";
+        char buf[32];
+        for (size_t i = 0; i < v.size(); ++i) {
+            snprintf(buf, sizeof buf, "%s%.6f", i ? ", " : "", v[i]);
+            s += buf;
+        }
+        return s + "]";
+    };
     printf("{\"n\": %lld, \"inserted\": %lld, \"build_s\": %.6f, \"force_sample_s\": %.6f, "
+           "\"build_reps_s\": %s, \"force_reps_s\": %s, "
            "\"nsample\": %lld, \"threads\": %d, \"theta\": %g, \"order\": \"%s\", \"ordered\": %lld, "
            "\"key_s\": %.6f}\n",
-           (long long)n, (long long)inserted, build_s, force_s, (long long)ns, omp_get_max_threads(), theta,
-           hashed ? "key" : "index", (long long)norder, key_s);
-    delete tree;
+           (long long)n, (long long)inserted, median(build_s), median(force_s), list(build_s).c_str(),
+           list(force_s).c_str(), (long long)ns, omp_get_max_threads(), theta, hashed ? "key" : "index",
+           (long long)norder, key_s);
     delete[] bods;
+    return 0;
+}
+
+// Star ("sun") forces and the update of Interaction::interactBodies /
+// Interaction::updateBodies.  The loop over sun pairs and sun-body pairs is the
+// one of Interaction.cpp:111-124, run on one thread: the reference's OpenMP
+// loops there all add into the same sun's acceleration, and one thread gives
+// the order those additions are written in.  Every interaction is the
+// reference's own Interaction::singleInteraction, the update its own
+// Interaction::updateBodies.
+int cmd_stars(const char *in, const char *out, double soft, int time_step)
+{
+    FILE *f = fopen(in, "rb");
+    if (!f) return 2;
+    int64_t ns = 0, nb = 0;
+    if (fread(&ns, 8, 1, f) != 1 || fread(&nb, 8, 1, f) != 1) return 2;
+    std::vector<double> s[7], b[7];  // m, x, y, z, vx, vy, vz
+    for (auto &v : s)
+        if (!read_vec(f, v, ns)) return 2;
+    for (auto &v : b)
+        if (!read_vec(f, v, nb)) return 2;
+    fclose(f);
+    NUM_BODIES = (int)nb; NUM_SUNS = (int)ns; TIME_STEP = time_step; SOFTENING = soft;
+    Interaction inter(false);
+    LOGCFG.level = ERROR;
+    std::vector<Body> suns(ns), bods(nb);
+    auto load = [](std::vector<Body> &to, std::vector<double> *v) {
+        for (size_t i = 0; i < to.size(); ++i) {
+            to[i].mass = v[0][i];
+            to[i].position = Vector3D(v[1][i], v[2][i], v[3][i]);
+            to[i].velocity = Vector3D(v[4][i], v[5][i], v[6][i]);
+            to[i].acceleration = Vector3D(0.0, 0.0, 0.0);
+        }
+    };
+    load(suns, s);
+    load(bods, b);
+    omp_set_num_threads(1);
+    for (int si = 0; si < NUM_SUNS; si++) {
+        for (int ss = 0; ss < NUM_SUNS; ss++)
+            if (si != ss) inter.singleInteraction(&suns[si], &suns[ss], false, false);
+        for (int bi = 0; bi < NUM_BODIES; bi++) inter.singleInteraction(&suns[si], &bods[bi], true);
+    }
+    FILE *g = fopen(out, "wb");
+    if (!g) return 3;
+    auto put3 = [g](const std::vector<Body> &v, int what) {
+        for (const Body &q : v) {
+            const Vector3D &w = what == 0 ? q.acceleration : what == 1 ? q.position : q.velocity;
+            double a[3] = {w.x, w.y, w.z};
+            fwrite(a, 8, 3, g);
+        }
+    };
+    fwrite(&ns, 8, 1, g);
+    fwrite(&nb, 8, 1, g);
+    put3(suns, 0);
+    put3(bods, 0);
+    inter.updateBodies(suns.data(), bods.data());
+    put3(suns, 1);
+    put3(suns, 2);
+    put3(bods, 1);
+    put3(bods, 2);
+    fclose(g);
     return 0;
 }
 
@@ -259,8 +351,10 @@ int main(int argc, char **argv)
     if (c == "tree" && argc == 10)
         return cmd_tree(argv[2], argv[3], atof(argv[4]), atof(argv[5]), atof(argv[6]), atof(argv[7]),
                         atof(argv[8]), atof(argv[9]));
-    if (c == "time" && (argc == 6 || argc == 7))
-        return cmd_time(argv[2], atof(argv[3]), atof(argv[4]), atoll(argv[5]), argc == 7 ? atoi(argv[6]) : 0);
+    if (c == "time" && argc >= 6 && argc <= 8)
+        return cmd_time(argv[2], atof(argv[3]), atof(argv[4]), atoll(argv[5]), argc >= 7 ? atoi(argv[6]) : 0,
+                        argc == 8 ? atoi(argv[7]) : 1);
+    if (c == "stars" && argc == 6) return cmd_stars(argv[2], argv[3], atof(argv[4]), atoi(argv[5]));
     fprintf(stderr, "bad arguments\n");
     return 1;
 }

@@ -9,6 +9,9 @@ outputs as .npz:
   tree_*.npz   positions, masses, theta, softening, root octant -> the velocity
                increments of Tree::insert + Interaction::treeInteraction in SI
                units (Tree.cpp:71-278, Interaction.cpp:17-98) and the node dump
+  stars_*.npz  suns and bodies -> the velocity increments of the star loop
+               (Interaction.cpp:111-124) and the state after
+               Interaction::updateBodies (:219-255)
 
 Usage: make -C oracle ref && python tests/golden/make_golden.py
 """
@@ -71,6 +74,32 @@ def tree_inputs():
     return cases
 
 
+def star_inputs():
+    """Suns (m, x, y, z, vx, vy, vz) and bodies for the star loop, with
+    positions in AU and velocities in m/s as the reference keeps them; one body
+    sits on a sun (zero distance, left out)."""
+    r = np.random.default_rng(13)
+    cases = {}
+
+    def disk(n, m):
+        rad = r.uniform(0.1, 1.0, n); ph = r.uniform(0, 2 * np.pi, n)
+        x = rad * np.cos(ph); y = rad * np.sin(ph); z = r.normal(0, 0.01, n)
+        v = r.normal(0, 3e4, (3, n))
+        return [np.full(n, m) * r.uniform(0.5, 1.5, n), x, y, z, v[0], v[1], v[2]]
+
+    one = [np.array([1.0]), np.zeros(1), np.zeros(1), np.zeros(1), np.zeros(1), np.zeros(1), np.zeros(1)]
+    b = disk(400, 1e-6)
+    for k in range(1, 7):
+        b[k][17] = one[k][0] if k < 4 else b[k][17]
+    cases["stars_one"] = (one, b, 0.0, 3600)
+    two = [np.array([0.5, 0.5]), np.array([0.1, -0.1]), np.zeros(2), np.zeros(2),
+           np.zeros(2), np.array([-2.0e4, 2.0e4]), np.zeros(2)]
+    b = disk(300, 1e-7)
+    cases["stars_two"] = (two, b, 0.0, 3600)
+    cases["stars_two_soft"] = (two, b, 0.01 * O.TO_METERS, 3600)
+    return cases
+
+
 def main():
     if not O.have_ref():
         raise SystemExit("oracle/_ref/ref_harness missing: run `make -C oracle ref` first")
@@ -93,6 +122,13 @@ def main():
                             length=lroot, dv=dv, node_level=nodes["level"], node_external=nodes["external"],
                             node_d=nodes["d"])
         print(f"tree_{name}: n={len(x)} nodes={len(nodes)}")
+    for name, (suns, bods, soft_m, ts) in star_inputs().items():
+        out = O.ref_stars(suns, bods, soft_m, ts)
+        keys = ("m", "x", "y", "z", "vx", "vy", "vz")
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), softening_m=soft_m, time_step=ts,
+                            **{f"s_{k}": a for k, a in zip(keys, suns)}, **{f"b_{k}": a for k, a in zip(keys, bods)},
+                            **out)
+        print(f"{name}: suns={len(suns[0])} bodies={len(bods[0])}")
 
 
 if __name__ == "__main__":

@@ -61,6 +61,13 @@ def test_create_validates_before_device():
         bad = nns_amd.default_params(block_size=bs)
         assert L.nbx_create(C.byref(bad), 100, 1000, 0, C.byref(h)) == -1
         assert b"invalid parameters" in L.nbx_last_error(None)
+    for fw in (1, 3, 16, -2):  # force_waves: 0, 2, 4 or 8
+        bad = nns_amd.default_params(force_waves=fw)
+        assert L.nbx_create(C.byref(bad), 100, 1000, 0, C.byref(h)) == -1
+        assert b"invalid parameters" in L.nbx_last_error(None)
+        sizes = (C.c_int * 1)(100)
+        assert L.nbx_batch_create(C.byref(bad), 1, sizes, None, 0, C.byref(h)) == -1
+        assert b"force_waves" in L.nbx_batch_last_error(None)
     # the largest system: 47,662 samples of 1024 bodies fill the 2^20-entry top table
     assert L.nbx_create(C.byref(p), 48805889, 2 * 48805889 + 12000, 0, C.byref(h)) == -1
     assert b"maximum of 48,805,888" in L.nbx_last_error(None)
@@ -164,7 +171,8 @@ def test_runner_rejects_bad_options():
     import subprocess
     runner = os.path.join(os.path.dirname(nns_amd.__file__), "nbx_runner")
     assert os.access(runner, os.X_OK), "nns_amd/nbx_runner is not built (python -m nns_amd.build --runner)"
-    for args in (["-b", "100"], ["-b", "2048"], ["-n", "1"], ["-i", "0"], ["--no-such-option"], ["-n"]):
+    for args in (["-b", "100"], ["-b", "2048"], ["-n", "1"], ["-i", "0"], ["--no-such-option"], ["-n"],
+                 ["--suns", "9"], ["--suns", "-1"], ["-n", "3", "--suns", "2"], ["-vx"]):
         r = subprocess.run([runner, *args], capture_output=True, text=True, timeout=60)
         assert r.returncode != 0, args
     r = subprocess.run([runner, "-h"], capture_output=True, text=True, timeout=60)

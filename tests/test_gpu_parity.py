@@ -154,6 +154,42 @@ def test_pipeline_matches_oracle(oracle, kind, n):
     check_forces(bh, t, oracle, float(np.float32(np.float32(params.softening) ** 2)))
 
 
+@pytest.mark.parametrize("n,m", [(1000, 14000), (4097, 2 * 4097 + 12000), (70000, 70001)])
+def test_reset_arrays_post_state(n, m):
+    """nbx_reset_arrays leaves exactly what resetArraysKernel (Kernels.cu:21-64)
+    leaves, read back from arrays filled with garbage first: every child slot
+    -1; count 1 for bodies and 0 for cells; cell x, y, z, mass 0 and the
+    bodies' untouched; start -1; sorted 0; *index = n; *mutex and the six
+    bounding-box scalars 0."""
+    torch = _torch()
+    print(f"case reset n={n} m={m}", flush=True)
+    dev = torch.device("cuda")
+    r = np.random.default_rng(n)
+    f = {k: torch.as_tensor(r.normal(size=m).astype(np.float32), device=dev) for k in ("x", "y", "z", "mass")}
+    keep = {k: v[:n].cpu().numpy().copy() for k, v in f.items()}
+    ints = {k: torch.as_tensor(r.integers(-9, 9, size=m).astype(np.int32), device=dev)
+            for k in ("count", "start", "sorted")}
+    child = torch.as_tensor(r.integers(-9, 9, size=8 * m).astype(np.int32), device=dev)
+    sc = {k: torch.full((1,), 7, dtype=torch.int32, device=dev) for k in ("index", "mutex")}
+    bb = {k: torch.full((1,), 3.5, device=dev) for k in ("minX", "maxX", "minY", "maxY", "minZ", "maxZ")}
+    ctx = nns_amd.Context(n, m)
+    kw = nns_amd.KernelsWrapper(ctx)
+    kw.resetArrays(sc["mutex"], f["x"], f["y"], f["z"], f["mass"], ints["count"], ints["start"], ints["sorted"],
+                   child, sc["index"], bb["minX"], bb["maxX"], bb["minY"], bb["maxY"], bb["minZ"], bb["maxZ"], n, m)
+    torch.cuda.synchronize()
+    assert (child.cpu().numpy() == -1).all()
+    cnt = ints["count"].cpu().numpy()
+    assert (cnt[:n] == 1).all() and (cnt[n:] == 0).all()
+    for k in ("x", "y", "z", "mass"):
+        v = f[k].cpu().numpy()
+        assert np.array_equal(v[:n], keep[k]) and (v[n:] == 0).all(), k
+    assert (ints["start"].cpu().numpy() == -1).all()
+    assert (ints["sorted"].cpu().numpy() == 0).all()
+    assert sc["index"].item() == n and sc["mutex"].item() == 0
+    assert all(v.item() == 0.0 for v in bb.values())
+    ctx.check()
+
+
 def _cluster_case(core, n, seed):
     """`core` bodies in one tight cluster (sigma 1e-4 of an extent of about 2,
     so the cell holding them sits near level 12 and every cell above it holds
@@ -361,10 +397,12 @@ def test_update_bit_exact_and_step_matches_stages(oracle):
 
 
 def test_ten_step_run_tracks_oracle(oracle):
-    """Plain-method CPU case shape (star + disk, 10 steps) at reduced N: the GPU
-    trajectory and the oracle's FP64-force trajectory stay within 1e-5."""
+    """The plain-method CPU case (central star + disk, N = 32,768, 10 steps) with
+    the star inside the tree as an ordinary body: the GPU trajectory and the
+    oracle's FP64-force trajectory stay within 1e-5.  (The reference's own code
+    keeps the star outside the tree: tests/test_gpu_stars.py.)"""
     _torch()
-    n = 8192
+    n = 32768
     x, y, z, vx, vy, vz, m = make_case("star_disk", n, seed=4)
     p = nns_amd.default_params()
     bh = nns_amd.BarnesHut(x, y, z, vx, vy, vz, m, params=p)
@@ -419,6 +457,116 @@ def test_error_paths_do_not_fault(oracle):
     bh.update()
     bh.ctx.check()
     torch.cuda.synchronize()
+    # the tree is built over the bodies of the last bounding box only
+    a = bh.arr
+    kw, n, mm = bh.kw, bh.n, bh.m
+    kw.resetArrays(a["mutex"], a["x"], a["y"], a["z"], a["mass"], a["count"], a["start"], a["sorted"],
+                   a["child"], a["index"], a["minX"], a["maxX"], a["minY"], a["maxY"], a["minZ"], a["maxZ"], n, mm)
+    kw.computeBoundingBox(a["mutex"], a["x"], a["y"], a["z"], a["minX"], a["maxX"], a["minY"], a["maxY"],
+                          a["minZ"], a["maxZ"], n - 1)
+    with pytest.raises(nns_amd.NbxError) as e:
+        kw.buildTree(a["x"], a["y"], a["z"], a["mass"], a["count"], a["start"], a["child"], a["index"], a["minX"],
+                     a["maxX"], a["minY"], a["maxY"], a["minZ"], a["maxZ"], n, mm)
+    assert e.value.code == -1 and "bounding_box" in str(e.value)
+
+
+def _half_massless(m, x):
+    """Masses with every body of the half x < 0 at zero: each cell inside that
+    half has zero total mass, so many subtree jobs raise the latch."""
+    out = m.copy()
+    out[x < 0.0] = 0.0
+    return out
+
+
+@pytest.mark.parametrize("locked", [0, 1])
+def test_errors_raised_inside_job_queue_kernels(oracle, locked):
+    """Errors latched from inside the persistent job-queue kernels while other
+    workgroups of the same launch are still taking jobs: zero-mass cells in the
+    subtree summary (NBX_ERR_NONFINITE), and for the lock-based build a
+    capacity overflow inside the subtree insertion (NBX_ERR_CAPACITY).  Each
+    workgroup leaves as a whole; the check returns the code, once, and the
+    next step on valid input recovers."""
+    torch = _torch()
+    n = 60000
+    print(f"case latch inside job-queue kernels locked={locked} n={n}", flush=True)
+    params = nns_amd.default_params(build_locked=locked)
+    x, y, z, vx, vy, vz, m = make_case("plummer", n, seed=13)
+    bh = nns_amd.BarnesHut(x, y, z, vx, vy, vz, _half_massless(m, x), params=params)
+    with pytest.raises(nns_amd.NbxError) as e:
+        bh.update()
+        bh.ctx.check()
+    assert e.value.code == -5
+    bh.arr["mass"][:n] = torch.as_tensor(m, device=bh.dev)
+    bh.update()
+    bh.ctx.check()
+    if not locked:
+        return
+    # room for the top cells (more than 1024 bodies, wired before the subtrees)
+    # and 100 cells more: the insertion of the subtrees runs out
+    t = oracle.KeyTree(x, y, z, m)
+    ntop = int((t.count[1:] > 1024).sum())
+    assert t.ncell - 1 > ntop + 100
+    print(f"case locked capacity n={n} top cells={ntop}", flush=True)
+    bh = nns_amd.BarnesHut(x, y, z, vx, vy, vz, m, params=params, m=n + ntop + 100)
+    with pytest.raises(nns_amd.NbxError) as e:
+        bh.update()
+        bh.ctx.check()
+    assert e.value.code == -2
+    # still latched (no reset since): reported again, and the state is untouched
+    with pytest.raises(nns_amd.NbxError) as e:
+        bh.ctx.check()
+    assert e.value.code == -2
+
+
+def test_error_of_an_earlier_step_is_reported_later():
+    """A step whose summary latches an error leaves the bodies where they were
+    (the update stage does nothing); later steps clear the latch but keep the
+    error, which the next nbx_check reports once."""
+    torch = _torch()
+    n = 20000
+    print(f"case sticky error n={n}", flush=True)
+    x, y, z, vx, vy, vz, m = make_case("plummer", n, seed=15)
+    bh = nns_amd.BarnesHut(x, y, z, vx, vy, vz, m)
+    bh.update()
+    bh.update()
+    torch.cuda.synchronize()
+    keys = ("x", "y", "z", "vx", "vy", "vz")
+    before = {k: bh.host(k).copy() for k in keys}
+    bh.arr["mass"][:n] = torch.as_tensor(_half_massless(m, before["x"]), device=bh.dev)
+    bh.update()  # latches NONFINITE
+    torch.cuda.synchronize()
+    for k in keys:
+        assert np.array_equal(bh.host(k), before[k]), k
+    bh.arr["mass"][:n] = torch.as_tensor(m, device=bh.dev)
+    bh.update()
+    bh.update()
+    torch.cuda.synchronize()
+    assert not np.array_equal(bh.host("x"), before["x"])  # the later steps moved the bodies
+    with pytest.raises(nns_amd.NbxError) as e:
+        bh.ctx.check()
+    assert e.value.code == -5
+    bh.ctx.check()  # reported once
+
+
+def test_force_waves_parameter_reaches_the_context(oracle):
+    """nbx_params.force_waves sets the force kernel's waves per block (0 keeps
+    the current choice in nbx_set_params); groups are independent of the block
+    shape, so the accelerations are the same bit for bit."""
+    _torch()
+    ctx = nns_amd.Context(1000, params=nns_amd.default_params(force_waves=8))
+    assert ctx.get_params().force_waves == 8
+    ctx.set_params(nns_amd.default_params(force_waves=0))
+    assert ctx.get_params().force_waves == 8
+    ctx.set_params(nns_amd.default_params(force_waves=2))
+    assert ctx.get_params().force_waves == 2
+    ctx.close()
+    x, y, z, vx, vy, vz, m = make_case("plummer", 30000, seed=16)
+    acc = []
+    for w in (2, 4, 8):
+        bh = run_to_forces(x, y, z, vx, vy, vz, m, nns_amd.default_params(force_waves=w))
+        assert bh.ctx.get_params().force_waves == w
+        acc.append(np.stack([bh.host("ax"), bh.host("ay"), bh.host("az")]))
+    assert np.array_equal(acc[0], acc[1]) and np.array_equal(acc[1], acc[2])
 
 
 def test_valu_probe_chains_converge():

@@ -177,6 +177,64 @@ def test_reference_si_units_equal_plummer_g1(oracle, case):
     np.testing.assert_allclose(g["dv"], a * scale, rtol=1e-12, atol=0)
 
 
+STAR_CASES = ["stars_one", "stars_two", "stars_two_soft"]
+
+
+def _star_case(g):
+    keys = ("m", "x", "y", "z", "vx", "vy", "vz")
+    return [g[f"s_{k}"] for k in keys], [g[f"b_{k}"] for k in keys]
+
+
+@pytest.mark.parametrize("case", STAR_CASES)
+def test_star_forces_match_reference_bit_for_bit(oracle, case):
+    """The suns (bodies outside the tree, Interaction.cpp:111-124) and the
+    update (:219-255) restated: the reference's own velocity increments and
+    state after Interaction::updateBodies, bit for bit (one sun with a body on
+    it, a binary, a softened binary)."""
+    g = golden(f"{case}.npz")
+    suns, bods = _star_case(g)
+    ts, soft = float(g["time_step"]), float(g["softening_m"])
+    sa, ba = oracle.star_forces(suns[:4], bods[:4], kernel=0, softening_m=soft, time_step=ts)
+    assert np.array_equal(sa, g["sun_dv"]) and np.array_equal(ba, g["body_dv"])
+    sx = oracle.update_reference(*suns[1:], sa, ts)
+    bx = oracle.update_reference(*bods[1:], ba, ts)
+    assert np.array_equal(np.stack(sx[:3], 1), g["sun_x"]) and np.array_equal(np.stack(sx[3:], 1), g["sun_v"])
+    assert np.array_equal(np.stack(bx[:3], 1), g["body_x"]) and np.array_equal(np.stack(bx[3:], 1), g["body_v"])
+
+
+@pytest.mark.parametrize("case", ["stars_one", "stars_two"])
+def test_star_si_units_equal_plummer_g1(oracle, case):
+    """With zero softening the reference's SI increments of the star loop are
+    TIME_STEP * G / TO_METERS^2 times the G = 1 Plummer accelerations."""
+    g = golden(f"{case}.npz")
+    suns, bods = _star_case(g)
+    sa, ba = oracle.star_forces(suns[:4], bods[:4], kernel=1, eps2=0.0)
+    scale = float(g["time_step"]) * oracle.G_SI / oracle.TO_METERS ** 2
+    np.testing.assert_allclose(g["sun_dv"], sa * scale, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(g["body_dv"], ba * scale, rtol=1e-12, atol=0)
+
+
+def test_star_forces_closed_forms(oracle):
+    """One sun and one body: a_b = m_s d (d^2 + eps^2)^-3/2 towards the sun and
+    the opposite pull on the sun; for many suns and bodies the pair forces
+    cancel (sum of m a is zero) and a body on a sun feels nothing from it."""
+    eps2 = 0.01
+    a_s, a_b = oracle.star_forces(([2.0], [0.0], [0.0], [0.0]), ([3.0], [1.0], [0.0], [0.0]), eps2=eps2)
+    f = 1.0 / (1.0 + eps2) ** 1.5
+    assert a_b[0] == pytest.approx([-2.0 * f, 0.0, 0.0], rel=1e-15)
+    assert a_s[0] == pytest.approx([3.0 * f, 0.0, 0.0], rel=1e-15)
+    r = np.random.default_rng(44)
+    sm, bm = r.uniform(0.5, 1.0, 3), r.uniform(1e-6, 1e-5, 500)
+    sp, bp = r.normal(0, 0.2, (3, 3)), r.normal(0, 1, (3, 500))
+    bp[:, 5] = sp[:, 1]
+    a_s, a_b = oracle.star_forces((sm, *sp), (bm, *bp), eps2=eps2)
+    mom = (sm[:, None] * a_s).sum(0) + (bm[:, None] * a_b).sum(0)
+    assert np.abs(mom).max() <= 1e-14 * (sm[:, None] * np.abs(a_s)).sum()
+    d = sp[:, [0, 2]] - bp[:, [5]]  # body 5 sits on sun 1: only suns 0 and 2 pull it
+    w = sm[[0, 2]] / ((d * d).sum(0) + eps2) ** 1.5
+    np.testing.assert_allclose(a_b[5], (d * w).sum(1), rtol=1e-13)
+
+
 def test_key_tree_invariants_brute_force(oracle):
     r = np.random.default_rng(7)
     n = 700
