@@ -190,6 +190,17 @@ int nbx_star_forces(nbx_context *ctx, const float *sx, const float *sy, const fl
                     const float *mass, float *ax, float *ay, float *az, int n, float g, void *stream,
                     float *elapsed_ms);
 
+/* The sort of nbx_sort_keys (stable ascending LSD radix sort of n 64-bit keys into
+ * sorted_keys and perm) without a context and without allocating: no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory -- code under HIP
+ * graph capture, serving loops.  Its scratch is one static device array of the library
+ * (12.5 MiB on every GPU it runs on, held until the process ends), found on the current
+ * device at the first call there; n is at most 1,048,576 (NBX_ERR_INVALID above, message
+ * in nbx_last_error(NULL)).  keys, sorted_keys and perm are device arrays on the current
+ * device; stream (NULL = the null stream) must belong to it.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+int nbx_sort_keys_static(const uint64_t *keys, int n, uint64_t *sorted_keys, int32_t *perm, void *stream);
+
 /* Per-body interaction counts of the last tree (cells accepted plus leaf bodies other
  * than the body itself), written to counts[i] for body i.  Not part of a time step. */
 int nbx_count_interactions(nbx_context *ctx, int n, int64_t *counts, void *stream);

@@ -41,7 +41,7 @@ EXPORTS = [
     "nbx_ic_plummer", "nbx_ic_binary_disk", "nbx_autotune", "nbx_batch_create", "nbx_batch_destroy",
     "nbx_batch_last_error", "nbx_batch_bind", "nbx_batch_step", "nbx_batch_graph_step", "nbx_batch_check",
     "nbx_batch_count_interactions", "nbx_batch_autotune", "nbx_batch_copy_buffer", "nbx_force_vjp", "nbx_probe_valu",
-    "nbx_star_forces", "nbx_walk_stats", "nbx_batch_walk_stats",
+    "nbx_star_forces", "nbx_walk_stats", "nbx_batch_walk_stats", "nbx_sort_keys_static",
 ]
 
 
@@ -111,6 +111,7 @@ def lib():
     L.nbx_count_interactions.argtypes = [vp, i, vp, vp]
     L.nbx_star_forces.argtypes = [vp] + [vp] * 7 + [i] + [vp] * 7 + [i, f, vp, fp]
     L.nbx_walk_stats.argtypes = [vp, vp, vp]
+    L.nbx_sort_keys_static.argtypes = [vp, i, vp, vp, vp]
     L.nbx_batch_walk_stats.argtypes = [vp, vp, vp]
     L.nbx_force_vjp.argtypes = [vp, vp, vp, vp, f, vp, vp, vp, vp, vp, fp]
     L.nbx_step.argtypes = [vp, C.POINTER(Arrays), vp, fp]

@@ -30,7 +30,7 @@ COMMON = [
 ]
 
 SOURCES = ["k_basic.hip", "k_radix.hip", "k_tree.hip", "k_force.hip", "k_adjoint.hip", "nbx_api.cpp", "nbx_batch.cpp",
-           "nbx_ics.cpp", "k_probe.hip"]
+           "nbx_ics.cpp", "k_probe.hip", "nbx_static.hip"]
 
 
 def _compile(src: str) -> str:
