@@ -418,10 +418,7 @@ def main():
     torch.cuda.synchronize()
     i_alg1 = int(counts1.sum().item())
     i_alg = 0.5 * (i_alg0 + i_alg1)
-    try:  # the work the walk did for those interactions (lane slots, frames)
-        walk = bh.walk_stats()
-    except nns_amd.NbxError:
-        walk = None
+    walk = bh.walk_stats()  # the work the walk did for those interactions (lane slots, frames)
     ms_per_step = total_ms / K
     t_force = stage_ms["force"] * 1e-3
     ginter = i_alg / t_force / 1e9
@@ -522,14 +519,14 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": force_traffic(n, args.theta),
-                     "kernel": "force_ring_kernel" if walk else "force_kernel",
+                     "kernel": "force_kernel",
                      "flop_per_interaction": FLOP_PER_INTERACTION, **probe,
                      # counted on the device by the walk itself: one child step tests one child for
                      # the 64 lanes of a group; useful = accepted interactions / (64 x steps)
-                     **({"useful_lane_fraction": round(walk["useful_lane_fraction"], 4),
-                         "work_inflation": round(walk["work_inflation"], 3),
-                         "child_steps_per_group": round(walk["child_steps"] / walk["groups"], 1),
-                         "frames_per_group": round(walk["frames"] / walk["groups"], 1)} if walk else {})},
+                     "useful_lane_fraction": round(walk["useful_lane_fraction"], 4),
+                     "work_inflation": round(walk["work_inflation"], 3),
+                     "child_steps_per_group": round(walk["child_steps"] / walk["groups"], 1),
+                     "frames_per_group": round(walk["frames"] / walk["groups"], 1)},
         "cpu_baseline": cpu,
         "accuracy": accuracy,
     }

@@ -38,6 +38,9 @@
 // froot[0].  Ensemble: group g comes from a table {first, count, system};
 // the walk starts at froot[system] (thresholds of the system's own theta and
 // extent are in the descriptors).
+//
+// The counting variant (COUNT) also records, per group, the child steps (one
+// step tests one child for the 64 lanes) and the frames of the walk.
 #include "nbx_device.h"
 
 namespace nbx {
@@ -49,7 +52,7 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
     const int32_t *__restrict__ perm, const int32_t *__restrict__ gorder, const DeviceState *st, float eps2,
     float G, float *__restrict__ ax,
     float *__restrict__ ay, float *__restrict__ az, int64_t *__restrict__ counts, int *__restrict__ err,
-    const int4 *__restrict__ groups)
+    const int4 *__restrict__ groups, int64_t *__restrict__ walk)
 {
     __shared__ uint4 stk[W][kForceStack];
     if (latched(st)) return;
@@ -77,7 +80,7 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
     float sx = 0.f, sy = 0.f, sz = 0.f;  // compensated totals
     float cx = 0.f, cy = 0.f, cz = 0.f;  // compensations
     float px = 0.f, py = 0.f, pz = 0.f;  // partials of the current tile
-    int64_t cnt = 0;
+    int64_t cnt = 0, nsteps = 0, nframes = 0;
 
     // One child step: every lane of `mask` meets child q with descriptor
     // {tag, crit}; returns the lanes that open it.  The monopole runs with the
@@ -114,6 +117,7 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
         const int rt = __builtin_amdgcn_readfirstlane(rd.x);
         const float rc = __int_as_float(__builtin_amdgcn_readfirstlane(rd.y));
         const uint64_t om = step(froot4[sys], rt, rc, live);
+        if (COUNT) nsteps += 1;
         if (om) {
             if (lane == 0) s[0] = make_uint4((unsigned)rt, 0u, (unsigned)om, (unsigned)(om >> 32));
             sp = 1;
@@ -140,6 +144,10 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
         const int blk = tag >> 4;
         const uint64_t mask = ((uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)e.w) << 32) |
                               (uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)e.z);
+        if (COUNT) {
+            nsteps += nch;
+            nframes += 1;
+        }
         const float4 *bq = fnode + 8 * (int64_t)blk;
         const int2 *bd = fmeta + 8 * (int64_t)blk;
         float4 q[8];
@@ -187,204 +195,6 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
         az[i] = G * sz;
         if (COUNT) counts[i] = cnt;
     }
-}
-
-// ---------------------------------------------------------------------------
-// LDS-staged walk.  The same frames, predicate and interactions as
-// force_kernel, with the frame blocks brought into LDS by the vector memory
-// unit (global_load_lds_dwordx4: 12 lanes x 16 B = one 192-byte block) at the
-// moment a frame is pushed, not when it is popped:
-//   A. every child of the frame is tested (|d|^2 + eps^2 against crit) and the
-//      results kept in registers;
-//   B. the opened children are pushed (stack entries first, then one block
-//      load each into the LDS ring slot of its stack position);
-//   C. the accepted children's monopoles are evaluated.
-// The next frame is either the last child pushed in B -- whose block load was
-// issued before the interactions of C, so C hides its latency -- or an entry
-// pushed earlier, whose block has long since arrived.  Vector loads complete
-// in order, so a frame waits only for the loads issued before it (vmcnt).
-// The ring has kRing slots per wave, indexed by stack position: the block of
-// an entry stays in LDS until the stack grows kRing entries above it; the push
-// that reuses the slot marks that entry (kEvicted), and its pop loads the block
-// again.  LDS per wave: stack 160 x 12 B + ring 16 x 192 B = 4,992 B, so a CU
-// still holds 32 waves.
-constexpr int kRing = 16;
-constexpr unsigned kEvicted = 0x80000000u;  // tags hold block << 4 | children < 2^31
-
-template <bool KAHAN, bool COUNT, bool SOFT, bool BATCH, int W>
-__global__ void __launch_bounds__(64 * W, COUNT ? 4 : 8) force_ring_kernel(
-    int n, int ngroups, const float4 *__restrict__ nodes4, const float4 *__restrict__ fnode,
-    const int2 *__restrict__ fmeta, const float4 *__restrict__ froot4, const int2 *__restrict__ frootm,
-    const int32_t *__restrict__ perm, const int32_t *__restrict__ gorder, const DeviceState *st, float eps2,
-    float G, float *__restrict__ ax, float *__restrict__ ay, float *__restrict__ az, int64_t *__restrict__ counts,
-    int *__restrict__ err, const int4 *__restrict__ groups, int64_t *__restrict__ walk)
-{
-    __shared__ unsigned s_tag[W][kForceStack];
-    __shared__ uint64_t s_mask[W][kForceStack];
-    __shared__ float4 s_ring[W][kRing][12];  // per slot: 8 x (x, y, z, m), then 8 x {tag, crit}
-    if (latched(st)) return;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int g = xcd_swizzle(blockIdx.x, gridDim.x) * W + wave;
-    if (g >= ngroups) return;
-
-    int first, cnt_bodies, sys;
-    if (BATCH) {
-        const int4 gr = groups[g];
-        first = gr.x;
-        cnt_bodies = gr.y;
-        sys = gr.z;
-    } else {
-        first = g * 64;
-        cnt_bodies = min(64, n - first);
-        sys = 0;
-    }
-    const bool valid = lane < cnt_bodies;
-    const int p = valid ? gorder[first + lane] : 0;
-    const float4 me = valid ? nodes4[p] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const uint64_t live = __ballot(valid);
-
-    float sx = 0.f, sy = 0.f, sz = 0.f;
-    float cx = 0.f, cy = 0.f, cz = 0.f;
-    float px = 0.f, py = 0.f, pz = 0.f;
-    int64_t cnt = 0, nsteps = 0, nframes = 0;
-
-    // block load of frame block `blk` into ring slot `slot` (12 lanes x 16 B)
-    auto fetch = [&](int blk, int slot) {
-        if (lane < 12) {
-            const void *src = lane < 8 ? (const void *)(fnode + 8 * (int64_t)blk + lane)
-                                       : (const void *)(fmeta + 8 * (int64_t)blk + 2 * (lane - 8));
-            // the LDS operand is named in its own address space (the host pass of
-            // a template otherwise drops the kernel's stub)
-            __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)&s_ring[wave][slot][0],
-                                             16, 0, 0);
-        }
-    };
-    auto interact = [&](float dx, float dy, float dz, float d2, float w, int tag, uint64_t takem) {
-        if (COUNT) {
-            if (__builtin_amdgcn_inverse_ballot_w64(takem)) cnt += ((tag & 15) == 0 && (tag >> 4) == p) ? 0 : 1;
-        }
-        if (!SOFT) takem &= __ballot(d2 > 0.f);
-        if (__builtin_amdgcn_inverse_ballot_w64(takem)) {
-            asm volatile("");
-            const float ri = __builtin_amdgcn_rsqf(d2);
-            const float f = w * ri * (ri * ri);
-            px = fmaf(f, dx, px);
-            py = fmaf(f, dy, py);
-            pz = fmaf(f, dz, pz);
-        }
-    };
-
-    int sp = 0;
-    {
-        const int2 rd = frootm[sys];
-        const int rt = __builtin_amdgcn_readfirstlane(rd.x);
-        const float rc = __int_as_float(__builtin_amdgcn_readfirstlane(rd.y));
-        const float4 q = froot4[sys];
-        const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
-        const float d2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-        const uint64_t ok = __ballot(rc < d2);
-        const uint64_t om = live & ~ok;
-        interact(dx, dy, dz, d2, q.w, rt, live & ok);
-        if (COUNT) nsteps += 1;
-        if (om) {
-            if (lane == 0) {
-                s_tag[wave][0] = (unsigned)rt;
-                s_mask[wave][0] = om;
-            }
-            fetch(rt >> 4, 0);
-            sp = 1;
-        }
-    }
-    constexpr int kTile = KAHAN ? 8 : 0x7fffffff;
-    constexpr unsigned kWaitVm = 0x0F70;    // s_waitcnt vmcnt(0): every block load issued so far has landed
-    constexpr unsigned kWaitLgkm = 0xC07F;  // s_waitcnt lgkmcnt(0): every LDS read issued so far is back
-    bool overflow = false;
-    do {
-      for (int t = 0; t < kTile; ++t) {
-        if (sp <= 0) break;
-        if (sp > kForceStack - 8) {  // a frame pops one entry and pushes at most 8
-            overflow = true;
-            sp = 0;
-            break;
-        }
-        sp = __builtin_amdgcn_readfirstlane(sp - 1);
-        const unsigned tw = (unsigned)__builtin_amdgcn_readfirstlane((int)s_tag[wave][sp]);
-        const uint64_t mask0 = s_mask[wave][sp];
-        const uint64_t mask = ((uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)(mask0 >> 32)) << 32) |
-                              (uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)mask0);
-        const int tag = (int)(tw & ~kEvicted);
-        const int slot = sp & (kRing - 1);
-        if (tw & kEvicted) fetch(tag >> 4, slot);  // its slot was reused further up the stack
-        __builtin_amdgcn_s_waitcnt(kWaitVm);
-        const int nch = tag & 15;
-        if (COUNT) {
-            nsteps += nch;
-            nframes += 1;
-        }
-        // A+B. every child tested, the opened ones pushed at once.  The first push
-        // takes this frame's own stack position, so its block load would land in
-        // the slot C still reads: that one load waits until after C.
-        float cdx[8], cdy[8], cdz[8], cd2[8];
-        int ctag[8];
-        uint64_t take[8];
-        int top = sp, first_blk = -1;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            take[j] = 0;
-            if (j >= nch) continue;
-            // one address for all lanes: LDS broadcasts; the values stay in VGPRs
-            const float4 q = s_ring[wave][slot][j];
-            const int2 md = reinterpret_cast<const int2 *>(&s_ring[wave][slot][8])[j];
-            const float crit = __int_as_float(md.y);
-            cdx[j] = q.x - me.x;
-            cdy[j] = q.y - me.y;
-            cdz[j] = q.z - me.z;
-            cd2[j] = fmaf(cdz[j], cdz[j], fmaf(cdy[j], cdy[j], fmaf(cdx[j], cdx[j], eps2)));
-            if (COUNT) ctag[j] = md.x;
-            const uint64_t ok = __ballot(crit < cd2[j]);
-            take[j] = mask & ok;
-            const uint64_t om = mask & ~ok;
-            if (om) {
-                if (lane == 0) {
-                    s_tag[wave][top] = (unsigned)md.x;
-                    s_mask[wave][top] = om;
-                    if (top >= kRing) s_tag[wave][top - kRing] |= kEvicted;
-                }
-                if (top == sp) first_blk = __builtin_amdgcn_readfirstlane(md.x) >> 4;
-                else fetch(md.x >> 4, top & (kRing - 1));
-                top = __builtin_amdgcn_readfirstlane(top + 1);
-            }
-        }
-        // C. the accepted monopoles (masses read from the slot again)
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            if (j >= nch) break;
-            const float w = s_ring[wave][slot][j].w;
-            interact(cdx[j], cdy[j], cdz[j], cd2[j], w, COUNT ? ctag[j] : 0, take[j]);
-        }
-        if (top > sp) {
-            __builtin_amdgcn_s_waitcnt(kWaitLgkm);  // C's reads of the slot are back
-            fetch(first_blk, slot);
-        }
-        sp = top;
-      }
-        if (KAHAN) {
-            kahan_add(sx, cx, px); kahan_add(sy, cy, py); kahan_add(sz, cz, pz);
-            px = py = pz = 0.f;
-        }
-    } while (sp > 0);
-    if (overflow && lane == 0) atomicOr(err, kErrStack);
-    if (!KAHAN) {
-        sx = px; sy = py; sz = pz;
-    }
-    if (valid) {
-        const int i = perm[p];
-        ax[i] = G * sx;
-        ay[i] = G * sy;
-        az[i] = G * sz;
-        if (COUNT) counts[i] = cnt;
-    }
     if (COUNT && walk && lane == 0) {
         walk[2 * (int64_t)g] = nsteps;
         walk[2 * (int64_t)g + 1] = nframes;
@@ -403,21 +213,15 @@ struct ForceArgs {
     int *err;
     const int4 *groups;
     int64_t *walk;  // counting variant: child steps and frames per group (nullable)
-    bool ring;      // LDS-staged walk (force_ring_kernel)
 };
 
 template <bool KAHAN, bool COUNT, bool SOFT, bool BATCH, int W>
 static void launch_one(const ForceArgs &a, hipStream_t s)
 {
     const int nblk = (a.ngroups + W - 1) / W;
-    if (a.ring)
-        force_ring_kernel<KAHAN, COUNT, SOFT, BATCH, W><<<nblk, 64 * W, 0, s>>>(
-            a.n, a.ngroups, a.nodes4, a.fnode, a.fmeta, a.froot4, a.frootm, a.perm, a.gorder, a.st, a.eps2, a.G, a.ax,
-            a.ay, a.az, a.counts, a.err, a.groups, a.walk);
-    else
-        force_kernel<KAHAN, COUNT, SOFT, BATCH, W><<<nblk, 64 * W, 0, s>>>(
-            a.n, a.ngroups, a.nodes4, a.fnode, a.fmeta, a.froot4, a.frootm, a.perm, a.gorder, a.st, a.eps2, a.G, a.ax,
-            a.ay, a.az, a.counts, a.err, a.groups);
+    force_kernel<KAHAN, COUNT, SOFT, BATCH, W><<<nblk, 64 * W, 0, s>>>(
+        a.n, a.ngroups, a.nodes4, a.fnode, a.fmeta, a.froot4, a.frootm, a.perm, a.gorder, a.st, a.eps2, a.G, a.ax,
+        a.ay, a.az, a.counts, a.err, a.groups, a.walk);
 }
 
 template <bool KAHAN, bool COUNT, bool SOFT, bool BATCH>
@@ -447,7 +251,6 @@ static void dispatch(const ForceArgs &a, hipStream_t s, bool kahan, int waves)
 static ForceArgs single_args(nbx_context *c, float *ax, float *ay, float *az, float g, int64_t *counts)
 {
     ForceArgs a{};
-    a.ring = c->force_ring;
     a.n = c->n;
     a.ngroups = (c->n + 63) / 64;
     a.nodes4 = c->nodes4; a.fnode = c->fnode; a.fmeta = c->fmeta; a.froot4 = c->froot4; a.frootm = c->frootm;
@@ -479,7 +282,6 @@ void launch_force_waves(nbx_context *c, hipStream_t s, float *ax, float *ay, flo
 void launch_batch_force(nbx_batch *b, hipStream_t s, int64_t *counts, int waves, int64_t *walk)
 {
     ForceArgs a{};
-    a.ring = b->force_ring;
     a.walk = walk;
     a.n = b->N;
     a.ngroups = b->ngroups;

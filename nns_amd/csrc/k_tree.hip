@@ -273,8 +273,14 @@ __global__ void __launch_bounds__(256) subtree_insert_kernel(
         // launch may raise at any time) and takes the job; every wave reads its
         // decision from LDS after the barrier, so the workgroup continues or
         // leaves as a whole.  A wave that left on its own read would let the
-        // others pass the barrier and read an s_job nobody wrote.
-        if (tid == 0) s_job = latched(st) ? nsub : atomicAdd(&st->work, 1);
+        // others pass the barrier and read an s_job nobody wrote.  The job is
+        // taken whatever the latch says, so that the two requests travel
+        // together (one round trip per job); a job taken under the latch is
+        // dropped.
+        if (tid == 0) {
+            const int j = atomicAdd(&st->work, 1);
+            s_job = latched(st) ? nsub : j;
+        }
         __syncthreads();
         const int job = s_job;
         __syncthreads();
@@ -716,8 +722,12 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
     for (;;) {
         // One read of the latch per job, by thread 0, published with the job:
         // the workgroup continues or leaves as a whole (a zero cell mass raises
-        // the latch from inside this launch).
-        if (tid == 0) s_job = latched(st) ? nsub : atomicAdd(&st->work2, 1);
+        // the latch from inside this launch).  Both requests are issued at once,
+        // as in subtree_insert_kernel.
+        if (tid == 0) {
+            const int j = atomicAdd(&st->work2, 1);
+            s_job = latched(st) ? nsub : j;
+        }
         __syncthreads();
         const int job = s_job;
         __syncthreads();

@@ -115,7 +115,6 @@ int nbx_create(const nbx_params *p, int n_max, int m_max, int device, nbx_contex
     if (!c) return fail(nullptr, NBX_ERR_INVALID, "out of host memory");
     c->p = *p;
     c->force_waves = p->force_waves ? p->force_waves : 4;
-    if (const char *v = std::getenv("NBX_FORCE_RING")) c->force_ring = std::atoi(v) != 0;
     c->device = device;
     c->n_max = n_max;
     c->m_max = m_max;
@@ -419,7 +418,6 @@ int nbx_walk_stats(nbx_context *c, int64_t *out, void *stream)
 {
     if (!c || !out) return NBX_ERR_INVALID;
     if (c->stage < 4) return fail(c, NBX_ERR_ORDER, "nbx_walk_stats needs nbx_sort first");
-    if (!c->force_ring) return fail(c, NBX_ERR_INVALID, "walk statistics need the LDS-staged force walk");
     hipSetDevice(c->device);
     hipStream_t s = (hipStream_t)stream;
     const int n = c->n, ng = (n + 63) / 64;

@@ -106,7 +106,6 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
     b->N = (int)total;
     b->m = (int)m64;
     b->force_waves = p->force_waves == 2 || p->force_waves == 8 ? p->force_waves : 4;
-    if (const char *v = std::getenv("NBX_FORCE_RING")) b->force_ring = std::atoi(v) != 0;
     std::vector<int32_t> off(nsys + 1, 0), sys_of(total), samp(nsys + 1, 0);
     std::vector<float> th(nsys, p->theta);
     std::vector<int4> groups;
@@ -259,7 +258,6 @@ int nbx_batch_count_interactions(nbx_batch *b, int64_t *counts, void *stream)
 int nbx_batch_walk_stats(nbx_batch *b, int64_t *out, void *stream)
 {
     if (!b || !out) return NBX_ERR_INVALID;
-    if (!b->force_ring) return bfail(b, NBX_ERR_INVALID, "walk statistics need the LDS-staged force walk");
     hipSetDevice(b->device);
     hipStream_t s = (hipStream_t)stream;
     nbx::launch_batch_force(b, s, b->counts, b->force_waves, b->walk);

@@ -57,12 +57,16 @@ struct DeviceState {
     int seen;           // bits of err already reported by the host check (not moved to sticky)
     unsigned bbox_ticket;
     int nsub;           // subtrees found by the top-tree pass
-    int work;           // next subtree to build
-    int work2;          // next subtree to summarise
     int ntop;           // top cells (excluding roots)
     int topl[kLevels];  // top cells per level (lists in TreeScratch::top_list)
     int log_next;       // adjoint: next free record of the node-term log
     int log_count;      // adjoint: records written (sort length)
+    // The job counters of the subtree kernels take an atomic from every
+    // workgroup per job; they sit on a cache line of their own, so that the
+    // error latch, which the same workgroups read once per job, is not queued
+    // behind those atomics.
+    alignas(128) int work;  // next subtree to build
+    int work2;              // next subtree to summarise
 };
 
 // Tree-construction scratch shared by the single-system and ensemble paths.
@@ -141,7 +145,6 @@ struct nbx_context {
     int bbox_n = -1;
     bool fresh = false;  // arrays reset since the last build (root slots are -1)
     int force_waves = 4; // waves per block of the force kernel (2, 4 or 8; autotuned)
-    bool force_ring = false;     // LDS-staged force walk
     int64_t *walk = nullptr;     // [2 ceil(n_max / 64)] child steps and frames per force group
 };
 
@@ -156,7 +159,6 @@ struct nbx_batch {
     int ngroups = 0;
     std::string last_error;
     int force_waves = 4;
-    bool force_ring = false;
     int64_t *walk = nullptr;  // [2 ngroups]
     // device
     nbx::DeviceState *st = nullptr;

@@ -29,6 +29,10 @@
  *                        (Tree.cpp:71-97,229-277), child visiting order UNW..LSE
  *                        (Interaction.cpp:73-96) and the SI-unit force of
  *                        singleInteraction (Interaction.cpp:17-55).
+ *   orc_group_walk       work (child steps, frames) of the GPU's wavefront walk of the same
+ *                        lists for groups of up to 64 bodies; no reference counterpart (the
+ *                        reference walks one body at a time), pinned by the identities of a
+ *                        one-body group (steps = interactions + frames + 1 at theta <= 0.5).
  *   orc_direct_accel     FP64 direct sum, the closed form a_i = sum_j m_j d_ij (|d_ij|^2+eps^2)^-3/2.
  *   orc_direct_accel_fast the same sum vectorised over sources (summation order differs).
  *   orc_star_forces      the reference's suns: bodies outside the tree, direct and
@@ -428,6 +432,93 @@ int orc_tree_accel(const orc_tree *t, double theta, double eps2, int mac_mode,
         if (ninter) ninter[k] = cnt;
     }
     return 0;
+}
+
+/* ------------------------------------------------ work of the group walk */
+/* Bottom of the one-child chain that starts at cell c: the first cell on the way
+ * down with two or more non-empty children (a cell always holds two or more
+ * bodies, so a one-child cell's child is a cell). */
+static int64_t orc_chain_bottom(const orc_tree *t, int64_t c, int *nch)
+{
+    for (;;) {
+        int k = 0;
+        int32_t last = -1;
+        for (int s = 0; s < 8; ++s)
+            if (t->child[8 * c + s] != -1) { ++k; last = t->child[8 * c + s]; }
+        if (k == 1 && last <= -2) { c = -2 - (int64_t)last; continue; }
+        *nch = k;
+        return c;
+    }
+}
+
+/* Work of the wavefront walk that evaluates orc_tree_accel's mac_mode 1 lists
+ * for groups of up to 64 bodies at once (one lane per body).  The walk tests a
+ * cell child for all lanes of its parent's frame and opens it as a frame of
+ * its own for the lanes that reject it; one-child chains are described by
+ * their bottom cell and tested with its level's threshold (the chain's cells
+ * share mass and centre, and the threshold falls with the level, so each body
+ * accepts the chain or opens its bottom exactly as the per-body walk does).
+ * A frame therefore exists for every chain-bottom cell that at least one lane
+ * of the group opens -- the union of the lanes' opened sets -- and the child
+ * steps of a group are 1 (the root's test) plus the children of its frames.
+ * order[k] is the body in group-order position k; group g holds positions
+ * [first[g], first[g] + cnt[g]).  Outputs steps[g], frames[g].  Returns -2 for
+ * a tree of fewer than two bodies (it has no cell), -1 when out of memory. */
+int orc_group_walk(const orc_tree *t, double theta, double eps2, int64_t ngroups, const int32_t *order,
+                   const int64_t *first, const int32_t *cnt, int64_t *steps, int64_t *frames)
+{
+    if (t->n < 2) return -2;
+    const float crit0 = orc_crit0(t->lcell0, theta);
+    const float eps2f = (float)eps2;
+    float critf[ORC_KEY_LEVELS + 2];
+    for (int L = 0; L <= ORC_KEY_LEVELS + 1; ++L) critf[L] = ldexpf(crit0, -2 * L) + eps2f;
+    const int64_t nc = t->ncell;
+    int bad = 0;
+    #pragma omp parallel
+    {
+        uint8_t *mark = (uint8_t *)calloc((size_t)(nc ? nc : 1), 1);
+        int32_t *touched = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nc ? nc : 1));
+        if (!mark || !touched) {
+            #pragma omp atomic write
+            bad = 1;
+        }
+        #pragma omp for schedule(dynamic, 16)
+        for (int64_t g = 0; g < ngroups; ++g) {
+            if (!mark || !touched) continue;
+            int64_t nt = 0, st = 1, fr = 0;
+            for (int32_t l = 0; l < cnt[g]; ++l) {
+                const int32_t i = order[first[g] + l];
+                const float pxf = t->x[i], pyf = t->y[i], pzf = t->z[i];
+                int32_t stk[8 * (ORC_KEY_LEVELS + 2)];
+                int32_t sp = 0;
+                stk[sp++] = 0; /* the root cell, as a child of a virtual parent */
+                while (sp > 0) {
+                    int nch;
+                    const int64_t b = orc_chain_bottom(t, stk[--sp], &nch);
+                    const float dx = t->c32[4 * b] - pxf, dy = t->c32[4 * b + 1] - pyf,
+                                dz = t->c32[4 * b + 2] - pzf;
+                    const float d2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2f)));
+                    if (critf[t->level[b]] < d2) continue; /* accepted */
+                    if (!mark[b]) {
+                        mark[b] = 1;
+                        touched[nt++] = (int32_t)b;
+                        st += nch;
+                        fr += 1;
+                    }
+                    for (int s = 0; s < 8; ++s) {
+                        const int32_t ch = t->child[8 * b + s];
+                        if (ch <= -2) stk[sp++] = -2 - ch;
+                    }
+                }
+            }
+            for (int64_t k = 0; k < nt; ++k) mark[touched[k]] = 0;
+            steps[g] = st;
+            frames[g] = fr;
+        }
+        free(mark);
+        free(touched);
+    }
+    return bad ? -1 : 0;
 }
 
 /* ------------------------------------------------ vector-Jacobian product */

@@ -71,6 +71,9 @@ def lib():
     L.orc_direct_accel.argtypes = [C.c_int64, _f32p, _f32p, _f32p, _f32p, C.c_double, C.c_int64,
                                    C.c_void_p, _f64p]
     L.orc_tree_vjp.restype = C.c_int
+    L.orc_group_walk.restype = C.c_int
+    L.orc_group_walk.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int64, _i32p, C.c_void_p,
+                                 _i32p, C.c_void_p, C.c_void_p]
     L.orc_tree_vjp.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _f64p, _f64p, _f64p]
     L.orc_direct_accel_fast.restype = C.c_int
     L.orc_direct_accel_fast.argtypes = [C.c_int64, _f32p, _f32p, _f32p, _f32p, C.c_double, C.c_int64,
@@ -186,6 +189,29 @@ class KeyTree:
         if rc != 0:
             raise RuntimeError("orc_tree_accel failed")
         return acc.reshape(k, 3), cnt
+
+    def group_order(self, levels=10):
+        """Body ids in the wavefront walk's group order: key order, re-sorted
+        stably by the top `levels` Hilbert digits of the key."""
+        top = keys_hilbert(self.skeys & np.uint64((1 << 63) - 1)) >> np.uint64(3 * (21 - levels))
+        return self.perm[np.argsort(top, kind="stable")].astype(np.int32)
+
+    def group_walk(self, theta, eps2, order=None, group=64):
+        """Child steps and frames of the group walk (orc_group_walk) over groups of
+        `group` consecutive bodies of `order`; returns (steps[g], frames[g])."""
+        order = self.group_order() if order is None else np.ascontiguousarray(order, dtype=np.int32)
+        first = np.arange(0, len(order), group, dtype=np.int64)
+        cnt = np.minimum(group, len(order) - first).astype(np.int32)
+        steps = np.zeros(len(first), np.int64)
+        frames = np.zeros(len(first), np.int64)
+        p = lambda a: a.ctypes.data_as(C.c_void_p)
+        rc = lib().orc_group_walk(self._h, float(theta), float(eps2), len(first), order, p(first), cnt,
+                                  p(steps), p(frames))
+        if rc == -2:
+            raise ValueError("orc_group_walk: a tree of fewer than two bodies has no cell")
+        if rc != 0:
+            raise MemoryError("orc_group_walk: out of memory")
+        return steps, frames
 
     def vjp(self, theta, eps2, u, mac_mode=0):
         """dL/dx (n,3) and dL/dm (n) for L = sum_i u_i . a_i with the lists held fixed."""

@@ -288,6 +288,59 @@ def test_device_predicate_mode_agrees(oracle):
     assert (c0 != c1).mean() < 1e-2
 
 
+# ------------------------------------------------------- group walk work
+def _group_case(oracle, seed, n):
+    r = np.random.default_rng(seed)
+    x, y, z = (r.normal(0, 1, n).astype(np.float32) for _ in range(3))
+    x[: n // 4] *= 0.05  # a dense clump: deep cells and one-child chains
+    m = r.uniform(0.1, 1.0, n).astype(np.float32)
+    return oracle.KeyTree(x, y, z, m)
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.5])
+def test_group_walk_of_one_body_is_its_own_walk(oracle, theta):
+    """A group of one body tests exactly the children of its own walk, each of
+    them accepted (an interaction, or the body itself) or opened (a frame):
+    steps = interactions + frames + 1, the 1 being the body's own leaf, which it
+    always reaches for theta < 1/sqrt(3) (a cell holding the body is within
+    sqrt(3) of its side of the body, so it is never accepted)."""
+    t = _group_case(oracle, 11, 2500)
+    eps2 = float(np.float32(1e-3) ** 2)
+    order = t.group_order()
+    steps, frames = t.group_walk(theta, eps2, order=order, group=1)
+    _, cnt = t.accel(theta, eps2, mac_mode=1, targets=order)
+    assert (frames > 0).all()
+    assert np.array_equal(steps, cnt + frames + 1)
+
+
+def test_group_walk_wide_theta_and_unions(oracle):
+    """At theta = 1.2 a body may accept a cell that holds it and never reach its
+    own leaf (steps = interactions + frames + {0, 1}).  A group's frames are the
+    union of its bodies' opened cells: no fewer than any one body's, no more
+    than their sum; a group of one body repeated is that body's walk."""
+    t = _group_case(oracle, 12, 3000)
+    eps2 = float(np.float32(1e-3) ** 2)
+    order = t.group_order()
+    s1, f1 = t.group_walk(1.2, eps2, order=order, group=1)
+    _, cnt = t.accel(1.2, eps2, mac_mode=1, targets=order)
+    d = s1 - cnt - f1
+    assert set(np.unique(d)) <= {0, 1} and (d == 0).any()
+    s64, f64 = t.group_walk(1.2, eps2, order=order, group=64)
+    for g in range(len(f64)):
+        fg = f1[64 * g:64 * g + 64]
+        assert fg.max() <= f64[g] <= fg.sum()
+        assert s1[64 * g:64 * g + 64].max() <= s64[g]
+    assert (f64 < np.add.reduceat(f1, np.arange(0, len(f1), 64))).any()  # shared frames counted once
+    rep = np.repeat(order[:40], 64)
+    sr, fr = t.group_walk(1.2, eps2, order=rep, group=64)
+    assert np.array_equal(sr, s1[:40]) and np.array_equal(fr, f1[:40])
+    # the default group order is key order re-sorted by the top ten Hilbert digits
+    top = oracle.keys_hilbert(t.skeys & np.uint64((1 << 63) - 1)) >> np.uint64(33)
+    assert np.all(np.diff(top[np.argsort(t.perm)][order].astype(np.int64)) >= 0)
+    with pytest.raises(ValueError):
+        oracle.KeyTree(*(np.zeros(1, np.float32) for _ in range(4))).group_walk(0.5, eps2)
+
+
 # ------------------------------------------------------------ closed forms
 def test_direct_two_body_and_cube(oracle):
     eps = 0.1
