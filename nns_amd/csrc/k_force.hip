@@ -30,8 +30,15 @@
 // frames.  Bookkeeping that is not per child (stack bound, Kahan) is per frame:
 // the scalar unit, shared by the CU's four SIMDs, is the other limit here.
 //
-// Blocks are remapped so that the blocks one XCD receives (dispatch is
-// round-robin over the 8 XCDs) walk consecutive groups and share its L2.
+// Each XCD (dispatch is round-robin over the 8 XCDs) walks a range of
+// consecutive groups, so that neighbouring walks share its L2; the ranges are
+// sized by the child steps each group took in the previous launch, so that the
+// XCDs finish together (a split by group count leaves the launch waiting for
+// the XCD with the most work: 15 % over the mean for an ensemble of uneven
+// systems, 6 % for a centrally condensed sphere).
+// The launch's arguments that a group needs only when its walk starts or ends
+// are read from a job record in device memory (ForceJob), so that they hold
+// no scalar registers during the walk.
 //
 // Single system: group g is bodies [64g, 64g+64) of the group order (gorder:
 // sorted positions ordered by coarse Hilbert key) and the walk starts from
@@ -45,42 +52,36 @@
 
 namespace nbx {
 
-template <bool KAHAN, bool COUNT, bool SOFT, bool BATCH, int W>
-__global__ void __launch_bounds__(64 * W) force_kernel(
-    int n, int ngroups, const float4 *__restrict__ nodes4, const float4 *__restrict__ fnode,
-    const int2 *__restrict__ fmeta, const float4 *__restrict__ froot4, const int2 *__restrict__ frootm,
-    const int32_t *__restrict__ perm, const int32_t *__restrict__ gorder, const DeviceState *st, float eps2,
-    float G, float *__restrict__ ax,
-    float *__restrict__ ay, float *__restrict__ az, int64_t *__restrict__ counts, int *__restrict__ err,
-    const int4 *__restrict__ groups, int64_t *__restrict__ walk)
+// The walk of group g by the calling wave (all 64 lanes active), with the
+// wave's LDS stack s.  The launch's other arguments come from its job record
+// jb, read when the walk starts and again when it ends (the empty asm with a
+// memory clobber keeps the compiler from holding them through the walk).
+template <bool KAHAN, bool COUNT, bool SOFT, bool BATCH>
+static __device__ __forceinline__ void walk_group(int g, int lane, uint4 *s, const float4 *__restrict__ fnode,
+                                                  const int2 *__restrict__ fmeta, float eps2, const ForceJob *jb)
 {
-    __shared__ uint4 stk[W][kForceStack];
-    if (latched(st)) return;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int g = xcd_swizzle(blockIdx.x, gridDim.x) * W + wave;
-    if (g >= ngroups) return;
-
+    asm volatile("" ::: "memory");
     int first, cnt_bodies, sys;
     if (BATCH) {
-        const int4 gr = groups[g];
+        const int4 gr = jb->groups[g];
         first = gr.x;
         cnt_bodies = gr.y;
         sys = gr.z;
     } else {
         first = g * 64;
-        cnt_bodies = min(64, n - first);
+        cnt_bodies = min(64, jb->n - first);
         sys = 0;
     }
     const bool valid = lane < cnt_bodies;
-    const int p = valid ? gorder[first + lane] : 0;  // the lane's body, by sorted position
-    const float4 me = valid ? nodes4[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int p = valid ? jb->gorder[first + lane] : 0;  // the lane's body, by sorted position
+    const float4 me = valid ? jb->nodes4[p] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint64_t live = __ballot(valid);
 
     float sx = 0.f, sy = 0.f, sz = 0.f;  // compensated totals
     float cx = 0.f, cy = 0.f, cz = 0.f;  // compensations
     float px = 0.f, py = 0.f, pz = 0.f;  // partials of the current tile
     int64_t cnt = 0, nsteps = 0, nframes = 0;
+    int work = 1;  // child steps of the walk (the root's and every frame's children)
 
     // One child step: every lane of `mask` meets child q with descriptor
     // {tag, crit}; returns the lanes that open it.  The monopole runs with the
@@ -110,13 +111,12 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
         return omask;
     };
 
-    uint4 *s = stk[wave];
     int sp = 0;
     {
-        const int2 rd = frootm[sys];
+        const int2 rd = jb->frootm[sys];
         const int rt = __builtin_amdgcn_readfirstlane(rd.x);
         const float rc = __int_as_float(__builtin_amdgcn_readfirstlane(rd.y));
-        const uint64_t om = step(froot4[sys], rt, rc, live);
+        const uint64_t om = step(jb->froot4[sys], rt, rc, live);
         if (COUNT) nsteps += 1;
         if (om) {
             if (lane == 0) s[0] = make_uint4((unsigned)rt, 0u, (unsigned)om, (unsigned)(om >> 32));
@@ -144,6 +144,7 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
         const int blk = tag >> 4;
         const uint64_t mask = ((uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)e.w) << 32) |
                               (uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)e.z);
+        work += nch;
         if (COUNT) {
             nsteps += nch;
             nframes += 1;
@@ -184,44 +185,118 @@ __global__ void __launch_bounds__(64 * W) force_kernel(
             px = py = pz = 0.f;
         }
     } while (sp > 0);
-    if (overflow && lane == 0) atomicOr(err, kErrStack);
+    asm volatile("" ::: "memory");
+    if (overflow && lane == 0) atomicOr(jb->err, kErrStack);
     if (!KAHAN) {
         sx = px; sy = py; sz = pz;
     }
     if (valid) {
-        const int i = perm[p];
-        ax[i] = G * sx;
-        ay[i] = G * sy;
-        az[i] = G * sz;
-        if (COUNT) counts[i] = cnt;
+        const int i = jb->perm[p];
+        const float G = jb->G;
+        jb->ax[i] = G * sx;
+        jb->ay[i] = G * sy;
+        jb->az[i] = G * sz;
+        if (COUNT) jb->counts[i] = cnt;
     }
-    if (COUNT && walk && lane == 0) {
-        walk[2 * (int64_t)g] = nsteps;
-        walk[2 * (int64_t)g + 1] = nframes;
+    if (lane == 0) jb->gcost[g] = work;  // the next launch splits the groups by it
+    if (COUNT && jb->walk && lane == 0) {
+        jb->walk[2 * (int64_t)g] = nsteps;
+        jb->walk[2 * (int64_t)g + 1] = nframes;
+    }
+}
+
+// Groups per XCD at most: 1.25 times the mean, so that the launch has a fixed
+// size while the split follows the work.
+static inline int xcd_cap(int ngroups) { return std::max((ngroups + 7) / 8, (5 * ngroups + 31) / 32); }
+
+// Block b runs on XCD x = b mod 8 (dispatch is round-robin over the 8 XCDs) as
+// that XCD's block i = b / 8: its waves walk groups fcut[x] + W i + wave, so an
+// XCD walks consecutive groups (neighbouring walks share its L2) and its share
+// of the groups is the one the setup kernel chose; blocks past the share exit.
+template <bool KAHAN, bool COUNT, bool SOFT, bool BATCH, int W>
+__global__ void __launch_bounds__(64 * W) force_kernel(const float4 *__restrict__ fnode,
+                                                       const int2 *__restrict__ fmeta, float eps2, DeviceState *st)
+{
+    __shared__ uint4 stk[W][kForceStack];
+    if (latched(st)) return;
+    const int x = blockIdx.x & 7;
+    const int g = __builtin_amdgcn_readfirstlane(st->fcut[x] + (int)(blockIdx.x >> 3) * W + (int)(threadIdx.x >> 6));
+    if (g < st->fcut[x + 1])
+        walk_group<KAHAN, COUNT, SOFT, BATCH>(g, threadIdx.x & 63, stk[threadIdx.x >> 6], fnode, fmeta, eps2, &st->fj);
+}
+
+// A group's weight in the split: its recorded child steps, within [1, 2^24].
+static __device__ inline int64_t group_cost(int32_t c) { return min(max(c, 1), 1 << 24); }
+
+// Before each force launch: stores the launch's job record and splits its
+// groups over the 8 XCDs (a kernel, not a copy, so that the force stage can be
+// captured in a graph).  The split gives each XCD an equal share of the child
+// steps that the previous launch over the same groups counted (gcost), within
+// xcd_cap groups per XCD; without such a launch the shares are equal in groups.
+// Thread t sums the costs of groups [t C, t C + C), C a multiple of 4 (read as
+// int4), and the cuts fall on those chunk boundaries.  Every group is walked by
+// exactly one wave whatever the split, so the results do not depend on it.
+__global__ void __launch_bounds__(1024) force_setup_kernel(DeviceState *st, ForceJob jb, int cap)
+{
+    __shared__ int64_t part[1024];
+    __shared__ int ideal[9];
+    const int G = jb.ngroups, t = threadIdx.x;
+    const bool hist = st->fcost_n == G;
+    const int C = ((G + 1023) / 1024 + 3) & ~3;
+    const int a = min(G, t * C), b = min(G, a + C);
+    int64_t sum = 0;
+    if (hist) {
+        int k = a;
+        for (; k + 4 <= b; k += 4) {
+            const int4 v = *reinterpret_cast<const int4 *>(jb.gcost + k);
+            sum += group_cost(v.x) + group_cost(v.y) + group_cost(v.z) + group_cost(v.w);
+        }
+        for (; k < b; ++k) sum += group_cost(jb.gcost[k]);
+    }
+    part[t] = sum;
+    if (t < 9) ideal[t] = (int)((int64_t)G * t / 8);
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {  // inclusive scan of the per-thread sums
+        const int64_t v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    const int64_t total = part[1023], lo = part[t] - sum;
+    if (hist && total > 0)
+        for (int x = 1; x < 8; ++x) {  // the chunk in which the prefix reaches x/8 of the steps
+            const int64_t target = total * x / 8;
+            if (target >= lo && target < part[t]) ideal[x] = target - lo < part[t] - target ? a : b;
+        }
+    __syncthreads();
+    if (t == 0) {
+        int cut = 0;
+        st->fcut[0] = 0;
+        for (int x = 0; x < 8; ++x) {  // monotone, at most cap per XCD, all G groups covered
+            const int lo_b = max(cut, G - (7 - x) * cap), hi_b = min(G, cut + cap);
+            cut = min(max(ideal[x + 1], lo_b), hi_b);
+            st->fcut[x + 1] = cut;
+        }
+        st->fcost_n = G;
+        st->fj = jb;
     }
 }
 
 struct ForceArgs {
-    int n, ngroups;
-    const float4 *nodes4, *fnode, *froot4;
-    const int2 *fmeta, *frootm;
-    const int32_t *perm, *gorder;
-    const DeviceState *st;
-    float eps2, G;
-    float *ax, *ay, *az;
-    int64_t *counts;
-    int *err;
-    const int4 *groups;
-    int64_t *walk;  // counting variant: child steps and frames per group (nullable)
+    ForceJob jb;
+    const float4 *fnode;
+    const int2 *fmeta;
+    DeviceState *st;
+    float eps2;
 };
 
 template <bool KAHAN, bool COUNT, bool SOFT, bool BATCH, int W>
 static void launch_one(const ForceArgs &a, hipStream_t s)
 {
-    const int nblk = (a.ngroups + W - 1) / W;
-    force_kernel<KAHAN, COUNT, SOFT, BATCH, W><<<nblk, 64 * W, 0, s>>>(
-        a.n, a.ngroups, a.nodes4, a.fnode, a.fmeta, a.froot4, a.frootm, a.perm, a.gorder, a.st, a.eps2, a.G, a.ax,
-        a.ay, a.az, a.counts, a.err, a.groups, a.walk);
+    const int cap = xcd_cap(a.jb.ngroups);
+    force_setup_kernel<<<1, 1024, 0, s>>>(a.st, a.jb, cap);
+    force_kernel<KAHAN, COUNT, SOFT, BATCH, W><<<8 * ((cap + W - 1) / W), 64 * W, 0, s>>>(a.fnode, a.fmeta, a.eps2,
+                                                                                      a.st);
 }
 
 template <bool KAHAN, bool COUNT, bool SOFT, bool BATCH>
@@ -236,7 +311,7 @@ template <bool BATCH>
 static void dispatch(const ForceArgs &a, hipStream_t s, bool kahan, int waves)
 {
     const bool soft = a.eps2 > 0.f;
-    if (a.counts) {
+    if (a.jb.counts) {
         if (soft) launch_w<false, true, true, BATCH>(a, s, waves);
         else launch_w<false, true, false, BATCH>(a, s, waves);
     } else if (kahan) {
@@ -251,14 +326,17 @@ static void dispatch(const ForceArgs &a, hipStream_t s, bool kahan, int waves)
 static ForceArgs single_args(nbx_context *c, float *ax, float *ay, float *az, float g, int64_t *counts)
 {
     ForceArgs a{};
-    a.n = c->n;
-    a.ngroups = (c->n + 63) / 64;
-    a.nodes4 = c->nodes4; a.fnode = c->fnode; a.fmeta = c->fmeta; a.froot4 = c->froot4; a.frootm = c->frootm;
-    a.perm = c->perm; a.gorder = c->gorder; a.st = c->st;
+    ForceJob &j = a.jb;
+    j.n = c->n;
+    j.ngroups = (c->n + 63) / 64;
+    j.nodes4 = c->nodes4; j.gorder = c->gorder; j.froot4 = c->froot4; j.frootm = c->frootm;
+    j.perm = c->perm;
+    j.G = g;
+    j.ax = ax; j.ay = ay; j.az = az; j.counts = counts;
+    j.err = &c->st->err;
+    j.gcost = c->gcost;
+    a.fnode = c->fnode; a.fmeta = c->fmeta; a.st = c->st;
     a.eps2 = layout_eps2(c->p);
-    a.G = g;
-    a.ax = ax; a.ay = ay; a.az = az; a.counts = counts;
-    a.err = &c->st->err;
     return a;
 }
 
@@ -268,7 +346,7 @@ void launch_force(nbx_context *c, hipStream_t s, float *ax, float *ay, float *az
     if (c->n <= 0) return;
     if (frame_layout_stale(c)) launch_frame_layout(c, s);  // thresholds of the current theta and eps
     ForceArgs a = single_args(c, ax, ay, az, g, counts);
-    a.walk = walk;
+    a.jb.walk = walk;
     dispatch<false>(a, s, c->p.kahan != 0, c->force_waves);
 }
 
@@ -282,19 +360,22 @@ void launch_force_waves(nbx_context *c, hipStream_t s, float *ax, float *ay, flo
 void launch_batch_force(nbx_batch *b, hipStream_t s, int64_t *counts, int waves, int64_t *walk)
 {
     ForceArgs a{};
-    a.walk = walk;
-    a.n = b->N;
-    a.ngroups = b->ngroups;
-    a.nodes4 = b->nodes4; a.fnode = b->fnode; a.fmeta = b->fmeta; a.froot4 = b->froot4; a.frootm = b->frootm;
-    a.perm = b->perm; a.gorder = b->gorder; a.st = b->st;
+    ForceJob &j = a.jb;
+    j.walk = walk;
+    j.n = b->N;
+    j.ngroups = b->ngroups;
+    j.nodes4 = b->nodes4; j.gorder = b->gorder; j.froot4 = b->froot4; j.frootm = b->frootm;
+    j.perm = b->perm;
+    j.G = b->p.gravity;
+    j.ax = counts ? b->acc_scratch : b->ax;
+    j.ay = counts ? b->acc_scratch + b->N : b->ay;
+    j.az = counts ? b->acc_scratch + 2 * (size_t)b->N : b->az;
+    j.counts = counts;
+    j.err = &b->st->err;
+    j.groups = b->groups;
+    j.gcost = b->gcost;
+    a.fnode = b->fnode; a.fmeta = b->fmeta; a.st = b->st;
     a.eps2 = layout_eps2(b->p);
-    a.G = b->p.gravity;
-    a.ax = counts ? b->acc_scratch : b->ax;
-    a.ay = counts ? b->acc_scratch + b->N : b->ay;
-    a.az = counts ? b->acc_scratch + 2 * (size_t)b->N : b->az;
-    a.counts = counts;
-    a.err = &b->st->err;
-    a.groups = b->groups;
     dispatch<true>(a, s, b->p.kahan != 0, waves);
 }
 

@@ -138,13 +138,14 @@ int nbx_create(const nbx_params *p, int n_max, int m_max, int device, nbx_contex
               dalloc(&c->pblk, m) == hipSuccess && dalloc(&c->bpar, n) == hipSuccess && dalloc(&c->counts, n) == hipSuccess &&
               dalloc(&c->acc_scratch, 3 * n) == hipSuccess && dalloc(&c->bodies4, n) == hipSuccess &&
               dalloc(&c->star_part, (size_t)nbx::kStarPartBlocks * NBX_MAX_STARS * 3) == hipSuccess &&
-              dalloc(&c->walk, 2 * ((n + 63) / 64)) == hipSuccess &&
+              dalloc(&c->walk, 2 * ((n + 63) / 64)) == hipSuccess && dalloc(&c->gcost, (n + 63) / 64) == hipSuccess &&
               dalloc(&c->gkey, n) == hipSuccess && dalloc(&c->gkey2, n) == hipSuccess &&
               dalloc(&c->gorder, n) == hipSuccess &&
               hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
     for (int k = 0; ok && k < 8; ++k) ok = hipEventCreate(&c->stage_ev[k]) == hipSuccess;
     if (ok) ok = nbx::alloc_tree_scratch(c->ts, n_max, 1, (n_max + nbx::kSubtreeMax - 1) / nbx::kSubtreeMax);
-    if (ok) ok = hipMemset(c->st, 0, sizeof(DeviceState)) == hipSuccess;
+    if (ok) ok = hipMemset(c->st, 0, sizeof(DeviceState)) == hipSuccess &&
+                 hipMemset(c->gcost, 0, sizeof(int32_t) * ((n + 63) / 64)) == hipSuccess;
     if (ok) ok = hipDeviceSynchronize() == hipSuccess;
     if (!ok) {
         nbx_destroy(c);
@@ -161,7 +162,8 @@ int nbx_destroy(nbx_context *c)
     hipDeviceSynchronize();
     void *bufs[] = {c->st, c->bbox_part, c->keys, c->skeys, c->kbuf, c->perm, c->vbuf, c->pos, c->hist,
                     c->cell_level, c->mass64, c->mom64, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->pblk, c->bpar, c->counts,
-                    c->acc_scratch, c->star_part, c->walk, c->bodies4, c->gkey, c->gkey2, c->gorder};
+                    c->acc_scratch, c->star_part, c->walk, c->gcost, c->bodies4, c->gkey, c->gkey2,
+                    c->gorder};
     for (void *b : bufs)
         if (b) hipFree(b);
     nbx::free_tree_scratch(c->ts);

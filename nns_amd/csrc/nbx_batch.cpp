@@ -129,11 +129,12 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
               dalloc(&b->mass64, M) && dalloc(&b->mom64, 3 * M) && dalloc(&b->nodes4, M) &&
               dalloc(&b->fnode, 8 * M) && dalloc(&b->fmeta, 8 * M) && dalloc(&b->froot4, S) &&
               dalloc(&b->frootm, S) && dalloc(&b->pblk, M) && dalloc(&b->bpar, N) && dalloc(&b->counts, N) && dalloc(&b->acc_scratch, 3 * N) && dalloc(&b->bodies4, N) &&
-              dalloc(&b->walk, 2 * groups.size()) &&
+              dalloc(&b->walk, 2 * groups.size()) && dalloc(&b->gcost, groups.size()) &&
               dalloc(&b->gkey, N) && dalloc(&b->gkey2, N) && dalloc(&b->gorder, N) &&
               dalloc(&b->samp_off, S + 1) && nbx::alloc_tree_scratch(b->ts, (int)N, nsys, samp[nsys]);
     for (int k = 0; ok && k < 8; ++k) ok = hipEventCreate(&b->stage_ev[k]) == hipSuccess;
     ok = ok && hipMemset(b->st, 0, sizeof(nbx::DeviceState)) == hipSuccess &&
+         hipMemset(b->gcost, 0, sizeof(int32_t) * groups.size()) == hipSuccess &&
          hipMemcpy(b->off, off.data(), 4 * (S + 1), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(b->sys_of, sys_of.data(), 4 * N, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(b->theta, th.data(), 4 * S, hipMemcpyHostToDevice) == hipSuccess &&
@@ -157,7 +158,7 @@ int nbx_batch_destroy(nbx_batch *b)
     void *bufs[] = {b->st, b->off, b->sys_of, b->theta, b->groups, b->ext, b->mm, b->keys, b->skeys, b->kbuf,
                     b->k2, b->k2buf, b->perm, b->perm1, b->vbuf, b->pos, b->sorted, b->hist, b->child, b->index,
                     b->count, b->start, b->cell_level, b->mass64, b->mom64, b->nodes4,
-                    b->fnode, b->fmeta, b->froot4, b->frootm, b->pblk, b->bpar, b->counts, b->acc_scratch, b->samp_off, b->bodies4, b->gkey, b->gkey2, b->gorder, b->walk};
+                    b->fnode, b->fmeta, b->froot4, b->frootm, b->pblk, b->bpar, b->counts, b->acc_scratch, b->samp_off, b->bodies4, b->gkey, b->gkey2, b->gorder, b->walk, b->gcost};
     for (void *q : bufs)
         if (q) hipFree(q);
     nbx::free_tree_scratch(b->ts);

@@ -48,6 +48,25 @@ constexpr int kMaxBodies = kMaxSamples * kSubtreeMax;
 // in an int: nodes stay below 2^27.
 constexpr int kMaxNodes = 1 << 27;
 
+// Arguments of a force launch that a group needs only when its walk starts or
+// ends.  A setup kernel stores them in device memory before the launch and the
+// walk reads them there, so that they hold no scalar registers during the walk.
+struct ForceJob {
+    const float4 *nodes4;   // bodies (x, y, z, m) by sorted position
+    const int32_t *gorder;  // group order: sorted positions
+    const float4 *froot4;   // root of each system as a child (x, y, z, m)
+    const int2 *frootm;     // and its descriptor
+    const int4 *groups;     // ensemble: {first, count, system} per group
+    const int32_t *perm;    // sorted position -> body
+    float *ax, *ay, *az;    // accelerations by body
+    int64_t *counts;        // counting variant: interactions per body
+    int64_t *walk;          // counting variant: child steps and frames per group (nullable)
+    int32_t *gcost;         // child steps per group, written by every launch
+    int *err;               // error latch
+    int n, ngroups;
+    float G;
+};
+
 struct DeviceState {
     double ext[4];      // xMin, yMin, zMin, maxSpan
     double root64[4];   // root mass, com x, y, z
@@ -67,6 +86,9 @@ struct DeviceState {
     // behind those atomics.
     alignas(128) int work;  // next subtree to build
     int work2;              // next subtree to summarise
+    alignas(128) ForceJob fj;  // the current force launch's job record
+    int fcut[9];               // its groups per XCD: XCD x walks [fcut[x], fcut[x + 1])
+    int fcost_n;               // groups of the launch that wrote ForceJob::gcost last
 };
 
 // Tree-construction scratch shared by the single-system and ensemble paths.
@@ -146,6 +168,7 @@ struct nbx_context {
     bool fresh = false;  // arrays reset since the last build (root slots are -1)
     int force_waves = 4; // waves per block of the force kernel (2, 4 or 8; autotuned)
     int64_t *walk = nullptr;     // [2 ceil(n_max / 64)] child steps and frames per force group
+    int32_t *gcost = nullptr;    // [ceil(n_max / 64)] child steps per force group of the last launch
 };
 
 // Ensemble of S independent systems (batched form).  System s owns bodies
@@ -160,6 +183,7 @@ struct nbx_batch {
     std::string last_error;
     int force_waves = 4;
     int64_t *walk = nullptr;  // [2 ngroups]
+    int32_t *gcost = nullptr; // [ngroups]
     // device
     nbx::DeviceState *st = nullptr;
     int32_t *off = nullptr, *sys_of = nullptr;
