@@ -100,7 +100,17 @@ static __device__ __forceinline__ void walk_group(int g, int lane, uint4 *s, con
             if (__builtin_amdgcn_inverse_ballot_w64(takem)) cnt += ((tag & 15) == 0 && (tag >> 4) == p) ? 0 : 1;
         }
         if (!SOFT) takem &= __ballot(d2 > 0.f);
-        if (__builtin_amdgcn_inverse_ballot_w64(takem)) {  // exec = takem: no select, skipped when empty
+        if (SOFT && !COUNT) {
+            // |d|^2 + eps^2 > 0 on every lane, so the monopole is computed for all
+            // lanes and zeroed by one select for those that do not take the child:
+            // no exec-mask branch, fewer scalar instructions per step
+            const float ri = __builtin_amdgcn_rsqf(d2);
+            float f = q.w * ri * (ri * ri);
+            f = __builtin_amdgcn_inverse_ballot_w64(takem) ? f : 0.f;
+            px = fmaf(f, dx, px);
+            py = fmaf(f, dy, py);
+            pz = fmaf(f, dz, pz);
+        } else if (__builtin_amdgcn_inverse_ballot_w64(takem)) {  // exec = takem: no select, skipped when empty
             asm volatile("");  // keeps the branch (no if-conversion into selects of the partials)
             const float ri = __builtin_amdgcn_rsqf(d2);
             const float f = q.w * ri * (ri * ri);
@@ -238,35 +248,49 @@ static __device__ inline int64_t group_cost(int32_t c) { return min(max(c, 1), 1
 // exactly one wave whatever the split, so the results do not depend on it.
 __global__ void __launch_bounds__(1024) force_setup_kernel(DeviceState *st, ForceJob jb, int cap)
 {
-    __shared__ int64_t part[1024];
+    __shared__ int64_t wsum[16];
     __shared__ int ideal[9];
     const int G = jb.ngroups, t = threadIdx.x;
     const bool hist = st->fcost_n == G;
     const int C = ((G + 1023) / 1024 + 3) & ~3;
     const int a = min(G, t * C), b = min(G, a + C);
     int64_t sum = 0;
-    if (hist) {
+    if (hist) {  // batches of 8 int4 loads, all issued before the first sum
         int k = a;
+        for (; k + 32 <= b; k += 32) {
+            int4 v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = *reinterpret_cast<const int4 *>(jb.gcost + k + 4 * i);
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                sum += group_cost(v[i].x) + group_cost(v[i].y) + group_cost(v[i].z) + group_cost(v[i].w);
+        }
         for (; k + 4 <= b; k += 4) {
             const int4 v = *reinterpret_cast<const int4 *>(jb.gcost + k);
             sum += group_cost(v.x) + group_cost(v.y) + group_cost(v.z) + group_cost(v.w);
         }
         for (; k < b; ++k) sum += group_cost(jb.gcost[k]);
     }
-    part[t] = sum;
+    // inclusive scan of the per-thread sums: within each wave by shuffles, then
+    // over the 16 wave totals
+    const int lane = t & 63, wave = t >> 6;
+    int64_t inc = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) wsum[wave] = inc;
     if (t < 9) ideal[t] = (int)((int64_t)G * t / 8);
     __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {  // inclusive scan of the per-thread sums
-        const int64_t v = t >= o ? part[t - o] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    const int64_t total = part[1023], lo = part[t] - sum;
+    if (t == 0)
+        for (int w = 1; w < 16; ++w) wsum[w] += wsum[w - 1];
+    __syncthreads();
+    inc += wave > 0 ? wsum[wave - 1] : 0;
+    const int64_t total = wsum[15], lo = inc - sum;
     if (hist && total > 0)
         for (int x = 1; x < 8; ++x) {  // the chunk in which the prefix reaches x/8 of the steps
             const int64_t target = total * x / 8;
-            if (target >= lo && target < part[t]) ideal[x] = target - lo < part[t] - target ? a : b;
+            if (target >= lo && target < inc) ideal[x] = target - lo < inc - target ? a : b;
         }
     __syncthreads();
     if (t == 0) {
