@@ -14,12 +14,16 @@ interactions are the per-particle interaction lists (cells accepted by
 l/D < theta plus leaf bodies) counted on the GPU outside the timed region.
 `roofline` prices the force kernel at 20 flop per interaction against the
 MI355X FP32 vector peak, and carries the measured issue rates of scalar
-v_fma_f32 and of v_rsq_f32 on this device (nbx_probe_valu); `cpu_baseline` times the reference's own OpenMP tree
+v_fma_f32 and of v_rsq_f32 on this device (nbx_probe_valu) and the walk's own
+counters (nbx_walk_stats: the fraction of lane slots that do counted work, the
+work inflation, child steps and frames per group); `cpu_baseline` times the reference's own OpenMP tree
 code (oracle/_ref/ref_harness) on this box's host cores on a bounded sample,
 with the bodies walked in key order (the reference's `hashed` path, the faster
 of its two; index order is reported beside it).  `accuracy` holds the error
 against an FP64 direct sum on a seeded sample, at theta 0.5 and, with the
-force time, at theta 0.25.
+force time, at theta 0.25.  `locked_build` times three steps of the same
+bodies with the reference's build scheme (atomic child-slot locking,
+build_locked = 1) instead of the default build.
 
 Beside the headline the line carries the other forms, each on its own
 workload:
@@ -419,6 +423,15 @@ def main():
     i_alg1 = int(counts1.sum().item())
     i_alg = 0.5 * (i_alg0 + i_alg1)
     walk = bh.walk_stats()  # the work the walk did for those interactions (lane slots, frames)
+    # the reference's build scheme, atomic child-slot locking (build_locked = 1), on the
+    # same bodies: median of 3 steps, then back to the default build
+    bh.ctx.set_params(nns_amd.default_params(theta=args.theta, kahan=0 if args.no_kahan else 1, build_locked=1))
+    st_locked = [bh.update(stage_times=True) for _ in range(3)]
+    bh.ctx.set_params(params)
+    torch.cuda.synchronize()
+    bh.ctx.check()
+    locked = {"build_ms": round(_median([t[2] for t in st_locked]), 4),
+              "ms_per_step": round(_median([sum(t) for t in st_locked]), 4), "steps": 3}
     ms_per_step = total_ms / K
     t_force = stage_ms["force"] * 1e-3
     ginter = i_alg / t_force / 1e9
@@ -514,6 +527,7 @@ def main():
                                f"N={n}, theta={args.theta}, eps=1e-3, Kahan={'off' if args.no_kahan else 'on'}",
                    "n": n, "theta": args.theta, "parallelism": "single GPU"},
         "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "locked_build": locked,
         "interactions_per_step": int(i_alg),
         "wall_s": round(wall, 3),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
