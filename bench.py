@@ -21,7 +21,9 @@ code (oracle/_ref/ref_harness) on this box's host cores on a bounded sample,
 with the bodies walked in key order (the reference's `hashed` path, the faster
 of its two; index order is reported beside it).  `accuracy` holds the error
 against an FP64 direct sum on a seeded sample, at theta 0.5 and, with the
-force time, at theta 0.25.  `locked_build` times three steps of the same
+force time, at theta 0.25.  `stage_roofline` prices every other stage at its
+algorithmic HBM bytes (stage_bytes) over its event time against 8 TB/s.
+`locked_build` times three steps of the same
 bodies with the reference's build scheme (atomic child-slot locking,
 build_locked = 1) instead of the default build.
 
@@ -56,7 +58,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (datasheet)
+HBM_PEAK_TBS = 8.0         # MI355X HBM3E (datasheet)
 FLOP_PER_INTERACTION = 20  # GPU Gems 3 convention
+
+
+def stage_bytes(n, m, c):
+    """Algorithmic HBM bytes of the non-force stages for n bodies, m nodes and c
+    cells: reset 44m + 16(m - n); bbox 12n; build = keys 36n + 8 radix passes of
+    24n + lcp, tree and key-ordered bodies 32n + 40c; centre of mass 16(n + c) +
+    80c; sort stage (sorted[], frame layout, group order) 8n + 56c + 24n + 8n +
+    4 radix passes of 24n; update 60n."""
+    return {"reset": 44 * m + 16 * (m - n), "bbox": 12 * n, "build": 36 * n + 8 * 24 * n + 32 * n + 40 * c,
+            "com": 16 * (n + c) + 80 * c, "sort": 8 * n + 56 * c + 24 * n + 8 * n + 4 * 24 * n, "update": 60 * n}
 
 
 def parse():
@@ -423,6 +436,11 @@ def main():
     i_alg1 = int(counts1.sum().item())
     i_alg = 0.5 * (i_alg0 + i_alg1)
     walk = bh.walk_stats()  # the work the walk did for those interactions (lane slots, frames)
+    cells = int(bh.arr["index"].item()) - n
+    sb = stage_bytes(n, bh.m, cells)
+    stage_roofline = {k: {"bytes": int(b), "achieved_tbs": round(b / (stage_ms[k] * 1e-3) / 1e12, 3),
+                          "frac": round(b / (stage_ms[k] * 1e-3) / 1e12 / HBM_PEAK_TBS, 4)}
+                      for k, b in sb.items()}
     # the reference's build scheme, atomic child-slot locking (build_locked = 1), on the
     # same bodies: median of 3 steps, then back to the default build
     bh.ctx.set_params(nns_amd.default_params(theta=args.theta, kahan=0 if args.no_kahan else 1, build_locked=1))
@@ -528,6 +546,7 @@ def main():
                    "n": n, "theta": args.theta, "parallelism": "single GPU"},
         "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "locked_build": locked,
+        "stage_roofline": {"bound": "hbm", "peak_tbs": HBM_PEAK_TBS, "cells": cells, **stage_roofline},
         "interactions_per_step": int(i_alg),
         "wall_s": round(wall, 3),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
