@@ -38,6 +38,15 @@ def make_case(kind, n, seed=1):
         base = r.normal(0, 1, (n, 3)).astype(np.float32)
         idx = np.repeat(np.arange(n), r.integers(1, 7, n))[:n]
         p = base[idx]
+    elif kind == "line":  # collinear bodies: one key axis only, long one-child chains
+        t = np.sort(r.uniform(-1, 1, n))
+        p = np.stack([t, 0.5 * t, -0.25 * t], 1).astype(np.float32)
+    elif kind == "far_clumps":  # two clumps 1e3 apart: empty space, long chains down to each clump
+        a = r.normal(0, 0.1, (n // 2, 3))
+        b = r.normal(0, 0.1, (n - n // 2, 3)) + 1e3
+        p = np.vstack([a, b]).astype(np.float32)
+    elif kind == "offset":  # a unit cloud at 1e4 from the origin: the extent, not the origin, sets the keys
+        p = (r.normal(0, 1, (n, 3)) + 1e4).astype(np.float32)
     else:
         raise ValueError(kind)
     z0 = np.zeros(n, np.float32)
@@ -126,7 +135,8 @@ def check_forces(bh, t, oracle, eps2, tol_rms=1e-5, tol_max=2e-3):
 
 
 CASES = [("star_disk", 4096), ("star_disk", 32768), ("plummer", 20000), ("binary_disk", 50000),
-         ("cluster", 9000), ("coincident", 5000), ("plummer", 2), ("plummer", 3), ("plummer", 130)]
+         ("cluster", 9000), ("coincident", 5000), ("plummer", 2), ("plummer", 3), ("plummer", 130),
+         ("line", 6000), ("far_clumps", 12000), ("offset", 8000)]
 
 
 @pytest.mark.parametrize("kind,n", CASES)
