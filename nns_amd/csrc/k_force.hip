@@ -9,26 +9,26 @@
 // comes in through the scalar cache with three s_load_dwordx16 once per frame.
 // The children are then stepped through with the data already in SGPRs, so
 // the only dependent memory access of a frame is that one load; frames live in
-// an LDS stack of {block, nch | level << 4, lane mask}.
+// an LDS stack of {tag = block << 4 | children, 0, lane mask low, lane mask high}.
 //
 // Each lane applies its own opening test to a cell child (l/D < theta written
 // as crit_L + eps^2 < |d|^2 + eps^2 with crit_L = (l_L/theta)^2, the reference
 // CPU test of 3D/OpenMP/OpenMP_NBody/src/Interaction.cpp:64-67; the left side
 // comes with the child's descriptor, the right side is the interaction's own
-// |d|^2 + eps^2, so the test costs one compare).  The test is a ballot, so lane masks stay in SGPRs:
-// lanes that accept evaluate the cell's monopole (under an exec mask of
-// mask & accept), lanes that reject push the child's frame with mask & ~accept.
-// Every lane therefore sees exactly its own per-particle interaction list; the
-// wave pays for the union of the 64 lists.  Bodies are
-// always direct (Interaction.cpp:60-63); a zero distance contributes nothing
-// (:28-30).
+// |d|^2 + eps^2, so the test costs one compare).  The test is a ballot, so lane
+// masks stay in SGPRs: lanes of mask & accept add the cell's monopole (softened:
+// computed on every lane and zeroed by a select on the others; unsoftened and
+// counting variants: under an exec mask of those lanes), lanes of mask & ~accept
+// push the child's frame.  Every lane therefore sees exactly its own
+// per-particle interaction list; the wave pays for the union of the 64 lists.
+// Bodies are always direct (Interaction.cpp:60-63); a zero distance contributes
+// nothing (:28-30).
 //
 // Interaction: a += m d (|d|^2 + eps^2)^-3/2, d = q - x (the 20-flop form of
 // GPU Gems 3 ch. 31), with the raw v_rsq_f32 (|d|^2 + eps^2 is never denormal
 // when eps > 0; the unsoftened variant masks d = 0).  With Kahan on, monopoles
 // go into a plain partial that is folded into a compensated total every 8
-// frames.  Bookkeeping that is not per child (stack bound, Kahan) is per frame:
-// the scalar unit, shared by the CU's four SIMDs, is the other limit here.
+// frames.  Bookkeeping that is not per child (stack bound, Kahan) is per frame.
 //
 // Each XCD (dispatch is round-robin over the 8 XCDs) walks a range of
 // consecutive groups, so that neighbouring walks share its L2; the ranges are
@@ -84,10 +84,8 @@ static __device__ __forceinline__ void walk_group(int g, int lane, uint4 *s, con
     int work = 1;  // child steps of the walk (the root's and every frame's children)
 
     // One child step: every lane of `mask` meets child q with descriptor
-    // {tag, crit}; returns the lanes that open it.  The monopole runs with the
-    // exec mask set to the lanes that take the child (no select; the whole
-    // block is skipped when none does); a body's crit is -1, so it is always
-    // taken and never opened.
+    // {tag, crit}; returns the lanes that open it.  A body's crit is -1, so it
+    // is always taken and never opened.
     auto step = [&](const float4 q, const int tag, const float crit, const uint64_t mask) -> uint64_t {
         const float dx = q.x - me.x;
         const float dy = q.y - me.y;
