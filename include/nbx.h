@@ -43,6 +43,11 @@
  * The stages must be called in the reference's order (reset, bounding box,
  * build, centre of mass, sort, forces, update) on one context; nbx_build_tree
  * binds the tree arrays it is given to the context for the stages after it.
+ * A context's calls must not overlap on the device: its stages share the
+ * context's buffers (keys, tree scratch, the force launch's job record and
+ * work counters), so a context is used from one stream at a time (work queued
+ * on another stream must be ordered after it by the caller); independent
+ * systems that run concurrently take a context each.
  */
 #ifndef NBX_H
 #define NBX_H
