@@ -750,23 +750,84 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
     }
 }
 
-// Top summary in one block: top cells deepest level first, then the roots.
-__global__ void __launch_bounds__(1024) top_summary_kernel(SumOut o, Segs g, int nsamples,
-                                                           const int32_t *__restrict__ top_list, DeviceState *st)
+// Top summary: the top cells level by level, deepest first, then the roots.  A
+// level's cells read only the level below.  A cell's sum gathers from up to 8
+// children in unrelated cache lines, and one CU's address pipeline takes such
+// gathers a few lanes at a time, so a level of more than kTopNarrow cells is
+// spread over kTopBlocks workgroups (cell i to workgroup i mod kTopBlocks) that
+// meet at a grid barrier after it; the narrower levels above it (a top cell's
+// parent is a top cell: a level never has more cells than the one below) and
+// the roots are summed by workgroup 0 alone, with workgroup barriers.
+// kTopBlocks workgroups of 256 threads fit on the device at once, so every
+// workgroup reaches every barrier; the wait is bounded all the same: one that
+// runs out latches kErrStall, and every wave still finishes.
+constexpr int kTopBlocks = 64;
+constexpr int kTopNarrow = 64;
+constexpr int kTopSpin = 1 << 20;  // polls of one barrier wait (about a second)
+
+// Grid barrier number `phase` (1, 2, ...) of the launch.  Every thread makes its
+// stores visible device-wide (release), thread 0 arrives and waits until every
+// workgroup has, and every thread then acquires what they wrote.
+__device__ inline void top_barrier(DeviceState *st, int phase, bool &stalled)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int target = phase * (int)gridDim.x;
+        __hip_atomic_fetch_add(&st->tsync, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; !stalled && __hip_atomic_load(&st->tsync, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target;
+             ++k) {
+            if (k == kTopSpin) {
+                atomicOr(&st->err, kErrStall);
+                stalled = true;  // no further waits in this workgroup
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+__global__ void __launch_bounds__(256) top_summary_kernel(SumOut o, Segs g, int nsamples,
+                                                          const int32_t *__restrict__ top_list, DeviceState *st)
 {
     __shared__ int s_cnt[kLevels];
     __shared__ int s_stop;
-    if (threadIdx.x == 0) s_stop = latched(st);  // one read for the whole block
-    if (threadIdx.x < kLevels) s_cnt[threadIdx.x] = st->topl[threadIdx.x];  // every level's count, read once
+    const int tid = threadIdx.x;
+    if (tid == 0) s_stop = latched(st);  // one read for the whole workgroup
+    if (tid < kLevels) s_cnt[tid] = st->topl[tid];  // every level's count, read once
     __syncthreads();
-    if (s_stop) return;
-    for (int L = kKeyLevels; L >= 1; --L) {
+    // Latched before this launch: every workgroup reads it so and skips the sums,
+    // but still meets every barrier (a latch raised by this launch's own sums may
+    // be seen by some workgroups only).
+    const bool stop = s_stop != 0;
+    bool stalled = false;
+    int L = kKeyLevels, phase = 0;
+    for (; L >= 1; --L) {
         const int cnt = s_cnt[L];
         if (cnt == 0) continue;  // uniform: no barrier for an empty level
-        for (int i = threadIdx.x; i < cnt; i += blockDim.x) summarise<4>(o, top_list[L * nsamples + i], st, false);
-        __syncthreads();
+        if (cnt <= kTopNarrow) break;
+        if (!stop)
+            for (int i = blockIdx.x + gridDim.x * tid; i < cnt; i += gridDim.x * blockDim.x)
+                summarise<4>(o, top_list[L * nsamples + i], st, false);
+        top_barrier(st, ++phase, stalled);
     }
-    for (int s = threadIdx.x; s < g.nseg; s += blockDim.x) summarise<4>(o, seg_root(g, s), st, !g.batch);
+    if (blockIdx.x == 0 && !stop) {
+        for (; L >= 1; --L) {
+            const int cnt = s_cnt[L];
+            if (cnt == 0) continue;
+            for (int i = tid; i < cnt; i += blockDim.x) summarise<4>(o, top_list[L * nsamples + i], st, false);
+            __syncthreads();
+        }
+        for (int s = tid; s < g.nseg; s += blockDim.x) summarise<4>(o, seg_root(g, s), st, !g.batch);
+    }
+    if (tid == 0) {  // the last workgroup out (every other one is past its last wait) resets the barrier
+        const int out = __hip_atomic_fetch_add(&st->tdone, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (out == (int)gridDim.x - 1) {
+            __hip_atomic_store(&st->tsync, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&st->tdone, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
 }
 
 // ------------------------------------------------------------------ layout
@@ -1013,7 +1074,7 @@ static void summarise_tree(Segs g, int nsamples, TreeScratch &ts, DeviceState *s
                            const SumOut &o)
 {
     subtree_summary_kernel<<<kPersistentBlocks, 256, 0, s>>>(o, ts.subcells, st);
-    top_summary_kernel<<<1, 1024, 0, s>>>(o, g, nsamples, ts.top_list, st);
+    top_summary_kernel<<<kTopBlocks, 256, 0, s>>>(o, g, nsamples, ts.top_list, st);
 }
 
 static inline int samples_of(int n) { return (n + kSubtreeMax - 1) / kSubtreeMax; }

@@ -221,6 +221,7 @@ int nbx_check(nbx_context *c, void *stream)
     if (err & nbx::kErrCapacity) return fail(c, NBX_ERR_CAPACITY, "tree needs more than m - n cells");
     if (err & nbx::kErrBucket) return fail(c, NBX_ERR_BUCKET, "more than 8 bodies share one key");
     if (err & nbx::kErrStack) return fail(c, NBX_ERR_STACK, "traversal stack overflow");
+    if (err & nbx::kErrStall) return fail(c, NBX_ERR_HIP, "top summary: its workgroups did not run together");
     return NBX_OK;
 }
 

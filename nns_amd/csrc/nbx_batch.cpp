@@ -245,6 +245,7 @@ int nbx_batch_check(nbx_batch *b, void *stream)
     if (err & nbx::kErrCapacity) return bfail(b, NBX_ERR_CAPACITY, "tree needs more cells than the ensemble holds");
     if (err & nbx::kErrBucket) return bfail(b, NBX_ERR_BUCKET, "more than 8 bodies share one key");
     if (err & nbx::kErrStack) return bfail(b, NBX_ERR_STACK, "traversal stack overflow");
+    if (err & nbx::kErrStall) return bfail(b, NBX_ERR_HIP, "top summary: its workgroups did not run together");
     return NBX_OK;
 }
 

@@ -33,6 +33,7 @@ constexpr int kErrCapacity = 1;
 constexpr int kErrBucket = 2;
 constexpr int kErrStack = 4;
 constexpr int kErrNonFinite = 8;
+constexpr int kErrStall = 16;  // a grid barrier's bounded wait ran out (workgroups not resident together)
 
 // Tree construction: cells holding more than kSubtreeMax bodies form the "top"
 // of the tree and are derived from the sorted keys; every child range of a top
@@ -86,6 +87,10 @@ struct DeviceState {
     // behind those atomics.
     alignas(128) int work;  // next subtree to build
     int work2;              // next subtree to summarise
+    // Grid barrier of the top summary: arrivals of its workgroups and their
+    // exits; the last workgroup to leave sets both back to 0.
+    alignas(128) int tsync;
+    int tdone;
     alignas(128) ForceJob fj;  // the current force launch's job record
     int fcut[9];               // its groups per XCD: XCD x walks [fcut[x], fcut[x + 1])
     int fcost_n;               // groups of the launch that wrote ForceJob::gcost last
