@@ -1,6 +1,6 @@
 // k_basic.hip -- streaming stages: reset, bounding box, keys, update.
 // All of them are HBM-bound: coalesced grid-stride loops, no LDS except the
-// bounding-box block reduction.
+// bounding-box block reduction and the Hilbert state table of the keys.
 #include "nbx_device.h"
 
 namespace nbx {
@@ -35,7 +35,6 @@ __global__ void __launch_bounds__(256) reset_kernel(
         st->sticky |= st->err & ~st->seen;  // an unreported error of this step waits for nbx_check
         st->seen = 0;
         st->err = 0;
-        st->bbox_ticket = 0u;
     }
 }
 
@@ -50,100 +49,124 @@ __device__ inline float wave_max(float v)
     return v;
 }
 
-// Bounding box: per-block min/max, then the last block to finish (ticket
-// counter) folds the partials and derives the FP64 extent of the key grid:
-// origin (xMin, yMin, zMin) and maxSpan = max |max - min| over the axes
-// (3D/OpenMP/OpenMP_NBody/src/Interaction.cpp:143-147).
-__global__ void __launch_bounds__(256) bbox_kernel(
-    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z, int n,
-    float *__restrict__ part, DeviceState *st, float *minX, float *maxX, float *minY, float *maxY,
-    float *minZ, float *maxZ)
+// Bounding box in two launches: every workgroup folds its grid-stride share of
+// the bodies into six partial extremes, part[6 b .. 6 b + 6); one workgroup
+// then folds the partials and derives the FP64 extent of the key grid: origin
+// (xMin, yMin, zMin) and maxSpan = max |max - min| over the axes
+// (3D/OpenMP/OpenMP_NBody/src/Interaction.cpp:143-147).  The launch boundary
+// makes the partials visible to the fold.  (A last-workgroup ticket in one
+// launch needs a device-scope release from every workgroup, and on this GPU
+// each such release writes back the XCD's L2.)
+
+// min x, max x, min y, max y, min z, max z of the workgroup, in v on thread 0.
+__device__ inline void block_extremes(float (&v)[6], float (*red)[4])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 6; k += 2) {
+        v[k] = wave_min(v[k]);
+        v[k + 1] = wave_max(v[k + 1]);
+    }
+    if (lane == 0)
+        for (int k = 0; k < 6; ++k) red[k][wave] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+            for (int k = 0; k < 6; k += 2) {
+                v[k] = fminf(v[k], red[k][w]);
+                v[k + 1] = fmaxf(v[k + 1], red[k + 1][w]);
+            }
+}
+
+// Folds bodies i = first, first + stride, ... < end into v (min x, max x, min y,
+// max y, min z, max z); true if a coordinate is not finite.  Four strides per
+// iteration, their twelve loads issued before any is used (min and max are
+// exact, so the order of the folds does not matter).
+__device__ inline bool fold_extremes(const float *__restrict__ x, const float *__restrict__ y,
+                                     const float *__restrict__ z, int first, int end, int stride, float (&v)[6])
+{
+    bool bad = false;
+    auto fold = [&](float xi, float yi, float zi) {
+        bad |= !(isfinite(xi) && isfinite(yi) && isfinite(zi));
+        v[0] = fminf(v[0], xi); v[1] = fmaxf(v[1], xi);
+        v[2] = fminf(v[2], yi); v[3] = fmaxf(v[3], yi);
+        v[4] = fminf(v[4], zi); v[5] = fmaxf(v[5], zi);
+    };
+    int i = first;
+    for (; i < end - 3 * stride; i += 4 * stride) {
+        float xs[4], ys[4], zs[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            xs[u] = x[i + u * stride];
+            ys[u] = y[i + u * stride];
+            zs[u] = z[i + u * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) fold(xs[u], ys[u], zs[u]);
+    }
+    for (; i < end; i += stride) fold(x[i], y[i], z[i]);
+    return bad;
+}
+
+__global__ void __launch_bounds__(256) bbox_kernel(const float *__restrict__ x, const float *__restrict__ y,
+                                                   const float *__restrict__ z, int n, float *__restrict__ part,
+                                                   DeviceState *st)
 {
     __shared__ float red[6][4];
-    __shared__ int s_last, s_bad;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    float a = INFINITY, b = -INFINITY, c = INFINITY, d = -INFINITY, e = INFINITY, f = -INFINITY;
-    int bad = 0;
-    for (int i = blockIdx.x * blockDim.x + tid; i < n; i += blockDim.x * gridDim.x) {
-        float xi = x[i], yi = y[i], zi = z[i];
-        bad |= !(isfinite(xi) && isfinite(yi) && isfinite(zi));
-        a = fminf(a, xi); b = fmaxf(b, xi);
-        c = fminf(c, yi); d = fmaxf(d, yi);
-        e = fminf(e, zi); f = fmaxf(f, zi);
-    }
-    if (tid == 0) s_bad = 0;
-    a = wave_min(a); b = wave_max(b); c = wave_min(c); d = wave_max(d); e = wave_min(e); f = wave_max(f);
-    __syncthreads();
-    if (bad) s_bad = 1;
-    if (lane == 0) {
-        red[0][wave] = a; red[1][wave] = b; red[2][wave] = c;
-        red[3][wave] = d; red[4][wave] = e; red[5][wave] = f;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        const int nw = blockDim.x >> 6;
-        for (int w = 1; w < nw; ++w) {
-            red[0][0] = fminf(red[0][0], red[0][w]); red[1][0] = fmaxf(red[1][0], red[1][w]);
-            red[2][0] = fminf(red[2][0], red[2][w]); red[3][0] = fmaxf(red[3][0], red[3][w]);
-            red[4][0] = fminf(red[4][0], red[4][w]); red[5][0] = fmaxf(red[5][0], red[5][w]);
+    const int tid = threadIdx.x;
+    float v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
+    const bool bad = fold_extremes(x, y, z, blockIdx.x * blockDim.x + tid, n, blockDim.x * gridDim.x, v);
+    if (__ballot(bad) && (tid & 63) == 0) atomicOr(&st->err, kErrNonFinite);
+    block_extremes(v, red);
+    if (tid == 0)
+        for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = v[k];
+}
+
+__global__ void __launch_bounds__(256) bbox_fold_kernel(const float *__restrict__ part, int nparts, DeviceState *st,
+                                                        float *minX, float *maxX, float *minY, float *maxY,
+                                                        float *minZ, float *maxZ)
+{
+    __shared__ float red[6][4];
+    float v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+        const float *q = part + 6 * b;
+        for (int k = 0; k < 6; k += 2) {
+            v[k] = fminf(v[k], q[k]);
+            v[k + 1] = fmaxf(v[k + 1], q[k + 1]);
         }
-        for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = red[k][0];
-        if (s_bad) atomicOr(&st->err, kErrNonFinite);
-        __threadfence();
-        unsigned t = atomicAdd(&st->bbox_ticket, 1u);
-        s_last = (t == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    // last block: fold all partials
-    a = INFINITY; b = -INFINITY; c = INFINITY; d = -INFINITY; e = INFINITY; f = -INFINITY;
-    for (int k = tid; k < (int)gridDim.x; k += blockDim.x) {
-        const float *q = part + 6 * k;
-        a = fminf(a, __hip_atomic_load(q + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        b = fmaxf(b, __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        c = fminf(c, __hip_atomic_load(q + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        d = fmaxf(d, __hip_atomic_load(q + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        e = fminf(e, __hip_atomic_load(q + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        f = fmaxf(f, __hip_atomic_load(q + 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    }
-    a = wave_min(a); b = wave_max(b); c = wave_min(c); d = wave_max(d); e = wave_min(e); f = wave_max(f);
-    __syncthreads();
-    if (lane == 0) {
-        red[0][wave] = a; red[1][wave] = b; red[2][wave] = c;
-        red[3][wave] = d; red[4][wave] = e; red[5][wave] = f;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        const int nw = blockDim.x >> 6;
-        for (int w = 1; w < nw; ++w) {
-            red[0][0] = fminf(red[0][0], red[0][w]); red[1][0] = fmaxf(red[1][0], red[1][w]);
-            red[2][0] = fminf(red[2][0], red[2][w]); red[3][0] = fmaxf(red[3][0], red[3][w]);
-            red[4][0] = fminf(red[4][0], red[4][w]); red[5][0] = fmaxf(red[5][0], red[5][w]);
-        }
-        *minX = red[0][0]; *maxX = red[1][0]; *minY = red[2][0];
-        *maxY = red[3][0]; *minZ = red[4][0]; *maxZ = red[5][0];
-        double xs = fabs((double)red[1][0] - (double)red[0][0]);
-        double ys = fabs((double)red[3][0] - (double)red[2][0]);
-        double zs = fabs((double)red[5][0] - (double)red[4][0]);
+    block_extremes(v, red);
+    if (threadIdx.x == 0) {
+        *minX = v[0]; *maxX = v[1]; *minY = v[2];
+        *maxY = v[3]; *minZ = v[4]; *maxZ = v[5];
+        double xs = fabs((double)v[1] - (double)v[0]);
+        double ys = fabs((double)v[3] - (double)v[2]);
+        double zs = fabs((double)v[5] - (double)v[4]);
         double sp = xs > ys ? xs : ys;
         sp = sp < zs ? zs : sp;
-        st->ext[0] = red[0][0]; st->ext[1] = red[2][0]; st->ext[2] = red[4][0]; st->ext[3] = sp;
-        st->bbox_ticket = 0u;
+        st->ext[0] = v[0]; st->ext[1] = v[2]; st->ext[2] = v[4]; st->ext[3] = sp;
     }
 }
 
 // Keys, and (x, y, z, m) packed as float4 for the one-line gather of the layout.
+// Hilbert keys map the Morton key through the state table in LDS; Morton keys
+// need neither the table nor the barrier after its fill.
+template <bool HILBERT>
 __global__ void __launch_bounds__(256) keys_kernel(
     const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
-    const float *__restrict__ mass, int n, const DeviceState *__restrict__ st, int key_type,
+    const float *__restrict__ mass, int n, const DeviceState *__restrict__ st,
     uint64_t *__restrict__ keys, float4 *__restrict__ bodies4)
 {
+    __shared__ uint8_t s_tab[HILBERT ? kHilbertTable : 1];
+    if (HILBERT) {
+        hilbert_table_fill(s_tab);
+        __syncthreads();
+    }
     const double ext[4] = {st->ext[0], st->ext[1], st->ext[2], st->ext[3]};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += blockDim.x * gridDim.x) {
         const float xi = x[i], yi = y[i], zi = z[i];
         uint64_t k = morton_key(xi, yi, zi, ext);
-        if (key_type == NBX_KEY_HILBERT) k = hilbert_of(k & ~(1ull << 63));
+        if (HILBERT) k = hilbert_top_lds(k & ~(1ull << 63), kKeyLevels, s_tab);
         keys[i] = k;
         if (mass) bodies4[i] = make_float4(xi, yi, zi, mass[i]);
     }
@@ -296,14 +319,18 @@ void launch_bbox(nbx_context *c, hipStream_t s, const float *x, const float *y, 
 {
     const int block = 256;
     int g = grid_for(n, block * 8, c->bbox_blocks);
-    bbox_kernel<<<g, block, 0, s>>>(x, y, z, n, c->bbox_part, c->st, minX, maxX, minY, maxY, minZ, maxZ);
+    bbox_kernel<<<g, block, 0, s>>>(x, y, z, n, c->bbox_part, c->st);
+    bbox_fold_kernel<<<1, block, 0, s>>>(c->bbox_part, g, c->st, minX, maxX, minY, maxY, minZ, maxZ);
 }
 
 void launch_keys(nbx_context *c, hipStream_t s, const float *x, const float *y, const float *z, const float *mass,
                  int n, int key_type, uint64_t *keys)
 {
     const int block = 256;
-    keys_kernel<<<grid_for(n, block, 65536), block, 0, s>>>(x, y, z, mass, n, c->st, key_type, keys, c->bodies4);
+    if (key_type == NBX_KEY_HILBERT)
+        keys_kernel<true><<<grid_for(n, block, 65536), block, 0, s>>>(x, y, z, mass, n, c->st, keys, c->bodies4);
+    else
+        keys_kernel<false><<<grid_for(n, block, 65536), block, 0, s>>>(x, y, z, mass, n, c->st, keys, c->bodies4);
 }
 
 void launch_update(hipStream_t s, int block, const DeviceState *st, float *x, float *y, float *z, float *vx, float *vy,
@@ -352,45 +379,45 @@ __global__ void __launch_bounds__(256) reset_batch_kernel(int S, int N, int *__r
     }
 }
 
-// One block per system: min/max over [off[s], off[s+1]) and the FP64 extent.
-__global__ void __launch_bounds__(256) bbox_batch_kernel(
-    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
-    const int32_t *__restrict__ off, float *__restrict__ mm, double *__restrict__ ext, DeviceState *st)
+// Ensemble: one workgroup per chunk of at most kBboxChunk bodies of one system
+// (chunk[c] = {system, first, end}) folds its partial extremes into part[6 c ..];
+// then one thread per system folds its chunks [choff[s], choff[s+1]) into
+// mm[6 s ..] and the FP64 extent ext[4 s ..].  A workgroup per system would
+// leave the largest system's loop setting the launch's time.
+__global__ void __launch_bounds__(256) bbox_batch_kernel(const float *__restrict__ x, const float *__restrict__ y,
+                                                         const float *__restrict__ z, const int4 *__restrict__ chunk,
+                                                         float *__restrict__ part, DeviceState *st)
 {
     __shared__ float red[6][4];
-    const int s = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int lo = off[s], hi = off[s + 1];
-    float a = INFINITY, b = -INFINITY, c = INFINITY, d = -INFINITY, e = INFINITY, f = -INFINITY;
-    int bad = 0;
-    for (int i = lo + tid; i < hi; i += blockDim.x) {
-        float xi = x[i], yi = y[i], zi = z[i];
-        bad |= !(isfinite(xi) && isfinite(yi) && isfinite(zi));
-        a = fminf(a, xi); b = fmaxf(b, xi);
-        c = fminf(c, yi); d = fmaxf(d, yi);
-        e = fminf(e, zi); f = fmaxf(f, zi);
-    }
-    if (__ballot(bad) && lane == 0) atomicOr(&st->err, kErrNonFinite);
-    a = wave_min(a); b = wave_max(b); c = wave_min(c); d = wave_max(d); e = wave_min(e); f = wave_max(f);
-    if (lane == 0) {
-        red[0][wave] = a; red[1][wave] = b; red[2][wave] = c;
-        red[3][wave] = d; red[4][wave] = e; red[5][wave] = f;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        const int nw = blockDim.x >> 6;
-        for (int w = 1; w < nw; ++w) {
-            red[0][0] = fminf(red[0][0], red[0][w]); red[1][0] = fmaxf(red[1][0], red[1][w]);
-            red[2][0] = fminf(red[2][0], red[2][w]); red[3][0] = fmaxf(red[3][0], red[3][w]);
-            red[4][0] = fminf(red[4][0], red[4][w]); red[5][0] = fmaxf(red[5][0], red[5][w]);
+    const int tid = threadIdx.x;
+    const int4 ck = chunk[blockIdx.x];
+    float v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
+    const bool bad = fold_extremes(x, y, z, ck.y + tid, ck.z, blockDim.x, v);
+    if (__ballot(bad) && (tid & 63) == 0) atomicOr(&st->err, kErrNonFinite);
+    block_extremes(v, red);
+    if (tid == 0)
+        for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = v[k];
+}
+
+__global__ void __launch_bounds__(256) bbox_batch_fold_kernel(int S, const int32_t *__restrict__ choff,
+                                                              const float *__restrict__ part, float *__restrict__ mm,
+                                                              double *__restrict__ ext)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    float v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
+    for (int c = choff[s]; c < choff[s + 1]; ++c)
+        for (int k = 0; k < 6; k += 2) {
+            v[k] = fminf(v[k], part[6 * c + k]);
+            v[k + 1] = fmaxf(v[k + 1], part[6 * c + k + 1]);
         }
-        for (int k = 0; k < 6; ++k) mm[6 * s + k] = red[k][0];
-        double xs = fabs((double)red[1][0] - (double)red[0][0]);
-        double ys = fabs((double)red[3][0] - (double)red[2][0]);
-        double zs = fabs((double)red[5][0] - (double)red[4][0]);
-        double sp = xs > ys ? xs : ys;
-        sp = sp < zs ? zs : sp;
-        ext[4 * s + 0] = red[0][0]; ext[4 * s + 1] = red[2][0]; ext[4 * s + 2] = red[4][0]; ext[4 * s + 3] = sp;
-    }
+    for (int k = 0; k < 6; ++k) mm[6 * s + k] = v[k];
+    double xs = fabs((double)v[1] - (double)v[0]);
+    double ys = fabs((double)v[3] - (double)v[2]);
+    double zs = fabs((double)v[5] - (double)v[4]);
+    double sp = xs > ys ? xs : ys;
+    sp = sp < zs ? zs : sp;
+    ext[4 * s + 0] = v[0]; ext[4 * s + 1] = v[2]; ext[4 * s + 2] = v[4]; ext[4 * s + 3] = sp;
 }
 
 __global__ void __launch_bounds__(256) keys_batch_kernel(
@@ -415,7 +442,8 @@ void launch_batch_reset(nbx_batch *b, hipStream_t s)
 
 void launch_batch_bbox(nbx_batch *b, hipStream_t s)
 {
-    bbox_batch_kernel<<<b->S, 256, 0, s>>>(b->x, b->y, b->z, b->off, b->mm, b->ext, b->st);
+    bbox_batch_kernel<<<b->nchunk, 256, 0, s>>>(b->x, b->y, b->z, b->bchunk, b->bpart, b->st);
+    bbox_batch_fold_kernel<<<(b->S + 255) / 256, 256, 0, s>>>(b->S, b->bchunk_off, b->bpart, b->mm, b->ext);
 }
 
 void launch_batch_keys(nbx_batch *b, hipStream_t s)

@@ -1029,9 +1029,12 @@ __global__ void __launch_bounds__(256) group_keys_kernel(int n, const uint64_t *
                                                          const int32_t *__restrict__ sys_of,
                                                          K *__restrict__ gkey, const DeviceState *st)
 {
+    __shared__ uint8_t s_tab[kHilbertTable];
+    hilbert_table_fill(s_tab);
+    __syncthreads();
     if (latched(st)) return;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += blockDim.x * gridDim.x) {
-        uint64_t h = hilbert_top(skeys[k] & ~(1ull << 63), kGroupLevels);
+        uint64_t h = hilbert_top_lds(skeys[k] & ~(1ull << 63), kGroupLevels, s_tab);
         if (sys_of) h |= (uint64_t)sys_of[perm[k]] << (3 * kGroupLevels);
         gkey[k] = (K)h;
     }

@@ -108,15 +108,20 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
     b->force_waves = p->force_waves == 2 || p->force_waves == 8 ? p->force_waves : 4;
     std::vector<int32_t> off(nsys + 1, 0), sys_of(total), samp(nsys + 1, 0);
     std::vector<float> th(nsys, p->theta);
-    std::vector<int4> groups;
+    std::vector<int4> groups, chunks;
+    std::vector<int32_t> choff(nsys + 1, 0);
     for (int s = 0; s < nsys; ++s) {
         off[s + 1] = off[s] + sys_n[s];
+        for (int f = off[s]; f < off[s + 1]; f += nbx::kBboxChunk)
+            chunks.push_back(make_int4(s, f, std::min(f + nbx::kBboxChunk, off[s + 1]), 0));
+        choff[s + 1] = (int32_t)chunks.size();
         for (int i = off[s]; i < off[s + 1]; ++i) sys_of[i] = s;
         if (sys_theta) th[s] = sys_theta[s];
         for (int g = 0; g < sys_n[s]; g += 64) groups.push_back(make_int4(off[s] + g, std::min(64, sys_n[s] - g), s, 0));
         samp[s + 1] = samp[s] + (sys_n[s] + nbx::kSubtreeMax - 1) / nbx::kSubtreeMax;
     }
     b->ngroups = (int)groups.size();
+    b->nchunk = (int)chunks.size();
     const size_t N = total, M = m64, S = nsys;
     const size_t tiles = (N + nbx::kSortTile - 1) / nbx::kSortTile;
     bool ok = dalloc(&b->st, 1) && dalloc(&b->off, S + 1) && dalloc(&b->sys_of, N) && dalloc(&b->theta, S) &&
@@ -131,7 +136,8 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
               dalloc(&b->frootm, S) && dalloc(&b->pblk, M) && dalloc(&b->bpar, N) && dalloc(&b->counts, N) && dalloc(&b->acc_scratch, 3 * N) && dalloc(&b->bodies4, N) &&
               dalloc(&b->walk, 2 * groups.size()) && dalloc(&b->gcost, groups.size()) &&
               dalloc(&b->gkey, N) && dalloc(&b->gkey2, N) && dalloc(&b->gorder, N) &&
-              dalloc(&b->samp_off, S + 1) && nbx::alloc_tree_scratch(b->ts, (int)N, nsys, samp[nsys]);
+              dalloc(&b->samp_off, S + 1) && dalloc(&b->bchunk, chunks.size()) && dalloc(&b->bchunk_off, S + 1) &&
+              dalloc(&b->bpart, 6 * chunks.size()) && nbx::alloc_tree_scratch(b->ts, (int)N, nsys, samp[nsys]);
     for (int k = 0; ok && k < 8; ++k) ok = hipEventCreate(&b->stage_ev[k]) == hipSuccess;
     ok = ok && hipMemset(b->st, 0, sizeof(nbx::DeviceState)) == hipSuccess &&
          hipMemset(b->gcost, 0, sizeof(int32_t) * groups.size()) == hipSuccess &&
@@ -140,6 +146,8 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
          hipMemcpy(b->theta, th.data(), 4 * S, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(b->groups, groups.data(), sizeof(int4) * groups.size(), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(b->samp_off, samp.data(), 4 * (S + 1), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(b->bchunk, chunks.data(), sizeof(int4) * chunks.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(b->bchunk_off, choff.data(), 4 * (S + 1), hipMemcpyHostToDevice) == hipSuccess &&
          hipDeviceSynchronize() == hipSuccess;
     if (!ok) {
         nbx_batch_destroy(b);
@@ -158,7 +166,8 @@ int nbx_batch_destroy(nbx_batch *b)
     void *bufs[] = {b->st, b->off, b->sys_of, b->theta, b->groups, b->ext, b->mm, b->keys, b->skeys, b->kbuf,
                     b->k2, b->k2buf, b->perm, b->perm1, b->vbuf, b->pos, b->sorted, b->hist, b->child, b->index,
                     b->count, b->start, b->cell_level, b->mass64, b->mom64, b->nodes4,
-                    b->fnode, b->fmeta, b->froot4, b->frootm, b->pblk, b->bpar, b->counts, b->acc_scratch, b->samp_off, b->bodies4, b->gkey, b->gkey2, b->gorder, b->walk, b->gcost};
+                    b->fnode, b->fmeta, b->froot4, b->frootm, b->pblk, b->bpar, b->counts, b->acc_scratch, b->samp_off, b->bodies4, b->gkey, b->gkey2, b->gorder, b->walk, b->gcost,
+                    b->bchunk, b->bchunk_off, b->bpart};
     for (void *q : bufs)
         if (q) hipFree(q);
     nbx::free_tree_scratch(b->ts);

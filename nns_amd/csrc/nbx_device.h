@@ -56,30 +56,30 @@ static __constant__ unsigned char c_hil[12][8] = {
     {2, 3, 5, 4, 1, 0, 6, 7}, {4, 5, 3, 2, 7, 6, 0, 1}, {0, 3, 1, 2, 7, 4, 6, 5},
     {2, 1, 3, 0, 5, 6, 4, 7}, {4, 7, 5, 6, 3, 0, 2, 1}, {6, 5, 7, 4, 1, 2, 0, 3}};
 
-// Lebesgue2Hilbert(lebesgue, 21) (SFC/src/Tree.cpp:82-104): all 21 digits mapped.
-static __device__ inline uint64_t hilbert_of(uint64_t lebesgue)
+// The state machine as one 96-byte table for a copy in shared memory: entry
+// 8 dir + cell holds 8 c_dir[dir][cell] + c_hil[dir][cell], i.e. the next
+// state's row offset and the digit.  Indexed per lane, the constant tables make
+// every level a dependent divergent memory read; from shared memory a level
+// costs one LDS read.  A kernel fills its copy with hilbert_table_fill, then
+// waits at a workgroup barrier before the first lookup.
+constexpr int kHilbertTable = 96;
+static __device__ inline void hilbert_table_fill(uint8_t *s_tab)
 {
-    uint64_t h = 0;
-    int dir = 0;
-    for (int lvl = kKeyLevels; lvl > 0; --lvl) {
-        int cell = (int)((lebesgue >> ((lvl - 1) * 3)) & 7u);
-        h = (h << 3) + c_hil[dir][cell];
-        dir = c_dir[dir][cell];
-    }
-    return h;
+    for (int t = threadIdx.x; t < kHilbertTable; t += blockDim.x)
+        s_tab[t] = (uint8_t)(8 * c_dir[t >> 3][t & 7] + c_hil[t >> 3][t & 7]);
 }
 
-// The first `levels` Hilbert digits of a Lebesgue key (the top 3*levels bits of
-// hilbert_of): the mapping runs from the top level down, so stopping early
-// leaves the coarse digits exact.
-static __device__ inline uint64_t hilbert_top(uint64_t lebesgue, int levels)
+// The first `levels` Hilbert digits of a Lebesgue key (Lebesgue2Hilbert,
+// SFC/src/Tree.cpp:82-104; levels = 21 maps all of them).  The mapping runs
+// from the top level down, so stopping early leaves the coarse digits exact.
+static __device__ inline uint64_t hilbert_top_lds(uint64_t lebesgue, int levels, const uint8_t *s_tab)
 {
     uint64_t h = 0;
-    int dir = 0;
+    int row = 0;
     for (int lvl = kKeyLevels; lvl > kKeyLevels - levels; --lvl) {
-        int cell = (int)((lebesgue >> ((lvl - 1) * 3)) & 7u);
-        h = (h << 3) + c_hil[dir][cell];
-        dir = c_dir[dir][cell];
+        const int e = s_tab[row + (int)((lebesgue >> ((lvl - 1) * 3)) & 7u)];
+        h = (h << 3) + (uint64_t)(e & 7);
+        row = e & ~7;
     }
     return h;
 }

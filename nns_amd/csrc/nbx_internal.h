@@ -39,6 +39,7 @@ constexpr int kErrStall = 16;  // a grid barrier's bounded wait ran out (workgro
 // of the tree and are derived from the sorted keys; every child range of a top
 // cell with 2..kSubtreeMax bodies is a subtree built by one workgroup.
 constexpr int kSubtreeMax = 1024;
+constexpr int kBboxChunk = 4096;  // bodies per workgroup of the ensemble's bounding box
 constexpr int kLevels = kKeyLevels + 1;  // levels 0..21 per sample in the top table
 // The top table (kLevels entries per sample of kSubtreeMax bodies) is scanned by
 // a two-level scan of at most 2^20 entries: the largest system, or ensemble
@@ -75,7 +76,6 @@ struct DeviceState {
     int err;            // latched kErr* bits of the current step (cleared by the reset stage)
     int sticky;         // unreported bits of earlier steps, moved here by the reset stage
     int seen;           // bits of err already reported by the host check (not moved to sticky)
-    unsigned bbox_ticket;
     int nsub;           // subtrees found by the top-tree pass
     int ntop;           // top cells (excluding roots)
     int topl[kLevels];  // top cells per level (lists in TreeScratch::top_list)
@@ -196,6 +196,10 @@ struct nbx_batch {
     int4 *groups = nullptr;  // {first sorted position, bodies, system, 0}
     double *ext = nullptr;   // 4 per system
     float *mm = nullptr;     // 6 per system
+    int nchunk = 0;                 // bounding-box chunks: at most kBboxChunk bodies of one system each
+    int4 *bchunk = nullptr;         // [nchunk] {system, first body, end, 0}
+    int32_t *bchunk_off = nullptr;  // [S+1] first chunk of each system
+    float *bpart = nullptr;         // [6 nchunk] partial extremes
     uint64_t *keys = nullptr, *skeys = nullptr, *kbuf = nullptr, *k2 = nullptr, *k2buf = nullptr;
     int32_t *perm = nullptr, *perm1 = nullptr, *vbuf = nullptr, *pos = nullptr, *sorted = nullptr;
     uint32_t *hist = nullptr;
