@@ -12,7 +12,7 @@
 // built from kRadixBits ballots (no atomics, no intra-wave waits), so equal
 // digits keep their order; the upsweep needs counts only and takes them with
 // LDS atomics.
-#include "nbx_internal.h"
+#include "nbx_device.h"
 
 namespace nbx {
 
@@ -61,32 +61,6 @@ __global__ void __launch_bounds__(kSortThreads) radix_upsweep(
         for (int w = 0; w < kSortThreads / 64; ++w) s += cnt[w][d];
         hist[(int64_t)d * ntiles + blockIdx.x] = s;
     }
-}
-
-// Block-wide exclusive scan of one value per thread (blockDim.x a multiple of 64,
-// at most 1024); returns the thread's exclusive prefix and writes the block total
-// to *total.  Each wave scans its 64 values with six shuffles, the wave totals
-// meet in sh[0 .. blockDim.x / 64): two barriers in all.
-__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh, uint32_t *total)
-{
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) sh[wave] = x;
-    __syncthreads();
-    uint32_t pre = 0, tot = 0;
-    for (int w = 0; w < nw; ++w) {
-        const uint32_t t = sh[w];
-        pre += w < wave ? t : 0u;
-        tot += t;
-    }
-    *total = tot;
-    __syncthreads();  // sh is free again for the caller
-    return pre + x - v;
 }
 
 // Exclusive scan of every digit row hist[d][0..ntiles) in place (one block per

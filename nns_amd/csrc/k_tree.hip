@@ -16,8 +16,9 @@
 //   2. every child range of a top cell with 2..kSubtreeMax bodies is a subtree
 //      built by ONE wave from the shared-digit counts of neighbouring keys:
 //      cell ids from a prefix sum, parents from previous-smaller-value queries,
-//      one plain store per child slot (subtree_build_kernel).  Its exact cell
-//      count is known first, so one atomicAdd reserves its block of cell ids.
+//      one plain store per child slot (subtree_build_kernel).  Every subtree's
+//      exact cell count is known first (from the shared-digit counts), and one
+//      scan of the counts gives each its block of cell ids.
 //      The lock-based insertion (compare-and-swap on child slots, -2 = locked,
 //      at workgroup scope; subtree_insert_kernel) builds the same cells when
 //      nbx_params.build_locked is set;
@@ -205,6 +206,74 @@ __global__ void __launch_bounds__(256) top_wire_kernel(
     }
 }
 
+// Cell ids of the subtrees.  A subtree [a, b) of sorted bodies with its root at
+// level L0 has one cell per run of two or more bodies sharing L digits, L0 <= L
+// <= 21; body a + i starts the runs of levels P_i < L <= l_i (l_i = lcp[a + i],
+// P_i = l_{i-1}, P_0 = L0 - 1; the last body starts none), so the subtree has
+//     (lcp[a] - L0 + 1) + sum over a < k < b - 1 of max(0, lcp[k] - lcp[k - 1])
+// cells.  subtree_cells_kernel counts them, one wave per subtree (a job the
+// builds reject counts 0); subtree_base_kernel turns the counts into the
+// exclusive sum cbase[] (cbase[nsub] the total) and sets *index past the last
+// subtree cell, so subtree j owns the ids from base0 + ntop + cbase[j] (after
+// the top cells).  Each build workgroup reads its base instead of taking it from
+// one shared counter: the device-scope atomics of a shared counter are
+// serialised, one per subtree.  A build that counts other than cbase[j + 1] -
+// cbase[j] cells for its subtree latches kErrCapacity and leaves it unbuilt, so
+// no two subtrees can share an id.
+__global__ void __launch_bounds__(256) subtree_cells_kernel(const int4 *__restrict__ sub, int sub_max,
+                                                            const uint8_t *__restrict__ lcp,
+                                                            uint32_t *__restrict__ cbase, const DeviceState *st)
+{
+    if (latched(st)) return;
+    const int nsub = min(__hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), sub_max);
+    const int lane = threadIdx.x & 63;
+    const int waves = (int)(gridDim.x * blockDim.x) >> 6;
+    for (int job = (int)(blockIdx.x * blockDim.x + threadIdx.x) >> 6; job < nsub; job += waves) {
+        const int4 jb = sub[job];
+        const int a = jb.x, cnt = jb.y - jb.x, L0 = jb.z;
+        int c = 0;
+        if (cnt >= 2 && cnt <= kSubtreeMax && L0 >= 1 && L0 <= kKeyLevels)
+            for (int k = a + lane; k < a + cnt - 1; k += 64)
+                c += k == a ? (int)lcp[a] - L0 + 1 : max(0, (int)lcp[k] - (int)lcp[k - 1]);
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+        if (lane == 0) cbase[job] = (uint32_t)c;
+    }
+}
+
+// One workgroup of 1024 threads: 16 counts per thread per pass.
+__global__ void __launch_bounds__(1024) subtree_base_kernel(uint32_t *__restrict__ cbase, int sub_max, int base0,
+                                                            int m, int *index, DeviceState *st)
+{
+    __shared__ uint32_t sh[16];
+    if (latched(st)) return;
+    const int nsub = min(__hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), sub_max);
+    constexpr int V = 16;
+    uint32_t carry = 0;
+    for (int t0 = 0; t0 < nsub; t0 += V * (int)blockDim.x) {
+        const int i0 = t0 + V * (int)threadIdx.x;
+        uint32_t v[V], s = 0;
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            v[u] = i0 + u < nsub ? cbase[i0 + u] : 0u;
+            s += v[u];
+        }
+        uint32_t tot;
+        uint32_t run = carry + block_exclusive_scan(s, sh, &tot);
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            if (i0 + u < nsub) cbase[i0 + u] = run;
+            run += v[u];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        cbase[nsub] = carry;  // so that subtree j has cbase[j + 1] - cbase[j] cells for every j
+        const int64_t end = (int64_t)base0 + st->ntop + carry;
+        if (end > m) atomicOr(&st->err, kErrCapacity);
+        else *index = (int)end;
+    }
+}
+
 __device__ inline int ld_slot_wg(const int *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -261,13 +330,15 @@ __device__ inline int block_sum(int v, int *red)
 // workgroup scope: only this workgroup touches the subtree.
 __global__ void __launch_bounds__(256) subtree_insert_kernel(
     int n, int m, const uint64_t *__restrict__ skeys, const int32_t *__restrict__ perm,
-    const uint64_t *__restrict__ keys, const uint8_t *__restrict__ lcp, int *child, int *index,
-    int32_t *__restrict__ cell_level, int4 *__restrict__ sub, int4 *__restrict__ subcells, DeviceState *st)
+    const uint64_t *__restrict__ keys, const uint8_t *__restrict__ lcp, int *child, int base0,
+    const uint32_t *__restrict__ cbase, int32_t *__restrict__ cell_level, int4 *__restrict__ sub, int sub_max,
+    int4 *__restrict__ subcells, DeviceState *st)
 {
     __shared__ int s_job, s_base, s_next, s_bad;
     __shared__ int red[4];
     const int tid = threadIdx.x;
-    const int nsub = __hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int nsub = min(__hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), sub_max);
+    const int top = base0 + __hip_atomic_load(&st->ntop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
         // Thread 0 alone reads the error latch (which other workgroups of this
         // launch may raise at any time) and takes the job; every wave reads its
@@ -296,8 +367,8 @@ __global__ void __launch_bounds__(256) subtree_insert_kernel(
         const int ks = block_sum(contrib, red);
         const int Lmax = block_max(deep, red);
         if (tid == 0) {
-            const int base = atomicAdd(index, ks);
-            s_bad = (int64_t)base + ks > (int64_t)m;
+            const int base = top + (int)cbase[job];  // ids reserved by subtree_base_kernel
+            s_bad = (int64_t)base + ks > (int64_t)m || ks != (int)(cbase[job + 1] - cbase[job]);
             if (s_bad) atomicOr(&st->err, kErrCapacity);
             s_base = base;
             s_next = 1;
@@ -428,8 +499,9 @@ __device__ inline int wave_exclusive_sum(int v, int *total)
 
 __global__ void __launch_bounds__(64) subtree_build_kernel(
     int m, const uint64_t *__restrict__ skeys, const int32_t *__restrict__ perm, const uint8_t *__restrict__ lcp,
-    int *__restrict__ child, int *index, int32_t *__restrict__ cell_level, int *__restrict__ start,
-    const int4 *__restrict__ sub, int sub_max, int4 *__restrict__ subcells, DeviceState *st)
+    int *__restrict__ child, int base0, const uint32_t *__restrict__ cbase, int32_t *__restrict__ cell_level,
+    int *__restrict__ start, const int4 *__restrict__ sub, int sub_max, int4 *__restrict__ subcells,
+    DeviceState *st)
 {
     constexpr int C = kSubtreeMax / 64;                   // bodies per lane at most
     __shared__ uint8_t s_min[kLiftLevels][kSubtreeMax];  // s_min[j][x] = min of l over [x, x + 2^j)
@@ -437,8 +509,11 @@ __global__ void __launch_bounds__(64) subtree_build_kernel(
     if (latched(st)) return;
     const int lane = threadIdx.x;
     const int nsub = min(__hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), sub_max);
+    const int top = base0 + __hip_atomic_load(&st->ntop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int job = blockIdx.x; job < nsub; job += gridDim.x) {
         const int4 jb = sub[job];
+        const uint32_t cb0 = cbase[job], cb1 = cbase[job + 1];
+        const int base = top + (int)cb0;  // ids reserved by subtree_base_kernel
         const int a = jb.x, cnt = jb.y - jb.x, L0 = jb.z, slot = jb.w;
         if (cnt < 2 || cnt > kSubtreeMax || L0 < 1 || L0 > kKeyLevels) {
             if (lane == 0) {
@@ -490,10 +565,7 @@ __global__ void __launch_bounds__(64) subtree_build_kernel(
             s_cs[i] = (uint16_t)run;
             run += max(0, (int)s_min[0][i] - P);
         }
-        int base = 0;
-        if (lane == 0) base = atomicAdd(index, ks);
-        base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
-        if (ks < 1 || ks > 65535 || (int64_t)base + ks > (int64_t)m) {
+        if (ks < 1 || ks > 65535 || ks != (int)(cb1 - cb0) || (int64_t)base + ks > (int64_t)m) {
             if (lane == 0) {
                 atomicOr(&st->err, kErrCapacity);
                 subcells[job] = make_int4(0, 0, L0, L0 - 1);
@@ -794,6 +866,9 @@ __global__ void __launch_bounds__(256) top_summary_kernel(SumOut o, Segs g, int 
     __shared__ int s_cnt[kLevels];
     __shared__ int s_stop;
     const int tid = threadIdx.x;
+    // The subtree summary before this launch is done with its job counter: set
+    // it back for a later summary of the same tree.
+    if (blockIdx.x == 0 && tid == 0) st->work2 = 0;
     if (tid == 0) s_stop = latched(st);  // one read for the whole workgroup
     if (tid < kLevels) s_cnt[tid] = st->topl[tid];  // every level's count, read once
     __syncthreads();
@@ -1065,12 +1140,16 @@ static void build_tree(Segs g, int nsamples, int base0, int m, TreeScratch &ts, 
     top_wire_kernel<<<(8 * wire + 255) / 256, 256, 0, s>>>(g, nsamples, skeys, perm, child, cell_level, ts.top_flag,
                                                         ts.top_id, ts.top_a, ts.top_b, ts.sub, ts.sub_max, ts.top_list,
                                                         base0, m, index, start, st);
+    subtree_cells_kernel<<<capped_grid(64 * (int64_t)ts.sub_max, 4096), 256, 0, s>>>(ts.sub, ts.sub_max, ts.lcp,
+                                                                                     ts.cbase, st);
+    subtree_base_kernel<<<1, 1024, 0, s>>>(ts.cbase, ts.sub_max, base0, m, index, st);
     if (locked)
-        subtree_insert_kernel<<<kPersistentBlocks, 256, 0, s>>>(n, m, skeys, perm, keys, ts.lcp, child, index,
-                                                                cell_level, ts.sub, ts.subcells, st);
+        subtree_insert_kernel<<<kPersistentBlocks, 256, 0, s>>>(n, m, skeys, perm, keys, ts.lcp, child, base0,
+                                                                ts.cbase, cell_level, ts.sub, ts.sub_max,
+                                                                ts.subcells, st);
     else
-        subtree_build_kernel<<<kBuildBlocks, 64, 0, s>>>(m, skeys, perm, ts.lcp, child, index, cell_level, start,
-                                                         ts.sub, ts.sub_max, ts.subcells, st);
+        subtree_build_kernel<<<kBuildBlocks, 64, 0, s>>>(m, skeys, perm, ts.lcp, child, base0, ts.cbase, cell_level,
+                                                         start, ts.sub, ts.sub_max, ts.subcells, st);
 }
 
 static void summarise_tree(Segs g, int nsamples, TreeScratch &ts, DeviceState *st, hipStream_t s,
@@ -1096,13 +1175,15 @@ bool alloc_tree_scratch(TreeScratch &ts, int n, int nseg, int samples)
            hipMalloc((void **)&ts.top_b, 4 * tt) == hipSuccess &&
            hipMalloc((void **)&ts.sub, sizeof(int4) * (size_t)ts.sub_max) == hipSuccess &&
            hipMalloc((void **)&ts.subcells, sizeof(int4) * (size_t)ts.sub_max) == hipSuccess &&
+           hipMalloc((void **)&ts.cbase, sizeof(uint32_t) * ((size_t)ts.sub_max + 1)) == hipSuccess &&
            hipMalloc((void **)&ts.top_list, 4 * tt) == hipSuccess &&
            hipMalloc((void **)&ts.scan_tmp, 4 * 1024) == hipSuccess && (int64_t)samples * kLevels <= (1 << 20);
 }
 
 void free_tree_scratch(TreeScratch &ts)
 {
-    void *p[] = {ts.lcp, ts.top_flag, ts.top_id, ts.top_a, ts.top_b, ts.sub, ts.subcells, ts.scan_tmp, ts.top_list};
+    void *p[] = {ts.lcp, ts.top_flag, ts.top_id, ts.top_a, ts.top_b, ts.sub, ts.subcells, ts.scan_tmp, ts.top_list,
+                 ts.cbase};
     for (void *q : p)
         if (q) hipFree(q);
     ts = TreeScratch{};

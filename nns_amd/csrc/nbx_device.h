@@ -84,6 +84,32 @@ static __device__ inline uint64_t hilbert_top_lds(uint64_t lebesgue, int levels,
     return h;
 }
 
+// Block-wide exclusive scan of one value per thread (blockDim.x a multiple of 64,
+// at most 1024); returns the thread's exclusive prefix and writes the block total
+// to *total.  Each wave scans its 64 values with six shuffles, the wave totals
+// meet in sh[0 .. blockDim.x / 64): two barriers in all.
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh, uint32_t *total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wave] = x;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (int w = 0; w < nw; ++w) {
+        const uint32_t t = sh[w];
+        pre += w < wave ? t : 0u;
+        tot += t;
+    }
+    *total = tot;
+    __syncthreads();  // sh is free again for the caller
+    return pre + x - v;
+}
+
 // ---------------------------------------------------------------- tree
 static __device__ inline int digit_at(uint64_t key, int level) { return (int)((key >> (60 - 3 * level)) & 7u); }
 

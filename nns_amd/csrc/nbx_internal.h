@@ -103,6 +103,7 @@ struct TreeScratch {
     int32_t *top_a = nullptr, *top_b = nullptr;       // [samples * kLevels] body range of a top cell
     int4 *sub = nullptr;         // {first, end, level, slot address}
     int4 *subcells = nullptr;    // {first cell id, cells, level, deepest level} of each subtree
+    uint32_t *cbase = nullptr;   // [sub_max + 1] cells of each subtree, then their exclusive sum
     int32_t *top_list = nullptr; // [kLevels * samples] top cells of each level
     uint32_t *scan_tmp = nullptr;  // 1024 block sums of the top-table scan
     int samples_max = 0, sub_max = 0;
