@@ -64,7 +64,9 @@ extern "C" {
 #define NBX_ERR_BUCKET (-3)    /* more than 8 bodies share one 63-bit key                */
 #define NBX_ERR_STACK (-4)     /* traversal stack overflow (cannot happen for depth <= 21) */
 #define NBX_ERR_NONFINITE (-5) /* NaN/Inf position or a zero/negative total mass          */
-#define NBX_ERR_HIP (-6)       /* HIP runtime error; see nbx_last_error                   */
+#define NBX_ERR_HIP (-6)       /* HIP runtime error, or the summary's workgroups could not
+                                  all run at once (its bounded grid-barrier wait ran out);
+                                  see nbx_last_error                                     */
 #define NBX_ERR_ORDER (-7)     /* stage called before the stage it depends on            */
 
 #define NBX_KEY_MORTON 0
