@@ -909,13 +909,15 @@ __global__ void __launch_bounds__(256) top_summary_kernel(SumOut o, Segs g, int 
 // Bodies in key order as float4 (x, y, z, m) at nodes4[0..n), made right after
 // the build for the summary; sorted[] (the reference's output of the sort
 // stage) in the sort stage; then the frame blocks of the force walk
-// (frame_layout_kernel).
-__global__ void __launch_bounds__(256) sorted_bodies_kernel(int n, const int32_t *__restrict__ perm,
+// (frame_layout_kernel).  The bodies are read in input order and stored at
+// their sorted positions pos[] (the sort's inverse permutation): random 16-byte
+// reads each fetch a whole line, random 16-byte stores only dirty their bytes.
+__global__ void __launch_bounds__(256) sorted_bodies_kernel(int n, const int32_t *__restrict__ pos,
                                                             const float4 *__restrict__ bodies4,
                                                             float4 *__restrict__ nodes4, const DeviceState *st)
 {
     if (latched(st)) return;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += blockDim.x * gridDim.x) nodes4[k] = bodies4[perm[k]];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += blockDim.x * gridDim.x) nodes4[pos[i]] = bodies4[i];
 }
 
 __global__ void __launch_bounds__(256) sorted_index_kernel(int n, const int32_t *__restrict__ perm,
@@ -1195,7 +1197,7 @@ void launch_build(nbx_context *c, hipStream_t s, int n, int m, int *child, int *
     Segs g{nullptr, nullptr, 1, n, 0};
     build_tree(g, samples_of(n), n, m, c->ts, c->st, s, n, c->skeys, c->perm, c->keys, child, index, c->cell_level,
                c->start, c->p.build_locked != 0);
-    sorted_bodies_kernel<<<capped_grid(n), 256, 0, s>>>(n, c->perm, c->bodies4, c->nodes4, c->st);
+    sorted_bodies_kernel<<<capped_grid(n), 256, 0, s>>>(n, c->pos, c->bodies4, c->nodes4, c->st);
 }
 
 void launch_summarize(nbx_context *c, hipStream_t s)
@@ -1232,7 +1234,7 @@ void launch_batch_build(nbx_batch *b, hipStream_t s)
     Segs g{b->off, b->samp_off, b->S, b->N, 1};
     build_tree(g, b->ts.samples_max, b->N + b->S, b->m, b->ts, b->st, s, b->N, b->skeys, b->perm, b->keys, b->child,
                b->index, b->cell_level, b->start, b->p.build_locked != 0);
-    sorted_bodies_kernel<<<capped_grid(b->N), 256, 0, s>>>(b->N, b->perm, b->bodies4, b->nodes4, b->st);
+    sorted_bodies_kernel<<<capped_grid(b->N), 256, 0, s>>>(b->N, b->pos, b->bodies4, b->nodes4, b->st);
 }
 
 void launch_batch_summarize(nbx_batch *b, hipStream_t s)
