@@ -273,14 +273,15 @@ template void launch_radix_passes<uint64_t>(uint32_t *, hipStream_t, const uint6
 template void launch_radix_passes<uint32_t>(uint32_t *, hipStream_t, const uint32_t *, const int32_t *, int, int, int,
                                             uint32_t *, int32_t *, uint32_t *, int32_t *, int32_t *, const int *);
 
-// Batched form: after the 8 key passes, a stable pass over the system index
-// groups every system's bodies, in key order, at [off[s], off[s+1]).
+// Batched form: after the 8 key passes, stable passes over the system index
+// (32-bit keys, one pass per 8 bits of it) group every system's bodies, in key
+// order, at [off[s], off[s+1]).
 __global__ void __launch_bounds__(256) system_key_kernel(int n, const int32_t *__restrict__ perm,
                                                          const int32_t *__restrict__ sys_of,
-                                                         uint64_t *__restrict__ k2)
+                                                         uint32_t *__restrict__ k2)
 {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += blockDim.x * gridDim.x)
-        k2[k] = (uint64_t)sys_of[perm[k]];
+        k2[k] = (uint32_t)sys_of[perm[k]];
 }
 
 __global__ void __launch_bounds__(256) gather_sorted_kernel(int n, const int32_t *__restrict__ perm,
@@ -303,9 +304,14 @@ void launch_batch_sort(nbx_batch *b, hipStream_t s)
     // 1. stable sort of (key, body) over the 63 key bits below the constant top bit
     launch_radix_passes(b->hist, s, b->keys, nullptr, n, 0, (kMortonSortBits + kRadixBits - 1) / kRadixBits,
                         b->skeys, b->perm1, b->kbuf, b->vbuf, nullptr);
-    // 2. stable sort by system (two passes cover 65536 systems)
-    system_key_kernel<<<g, 256, 0, s>>>(n, b->perm1, b->sys_of, b->k2);
-    launch_radix_passes(b->hist, s, b->k2, b->perm1, n, 0, 2, b->k2buf, b->perm, b->kbuf, b->vbuf, nullptr);
+    // 2. stable sort by system: 32-bit keys, one pass up to 256 systems, two up to 65536
+    int sys_bits = 0;
+    while ((1 << sys_bits) < b->S) ++sys_bits;
+    const int sys_passes = sys_bits > kRadixBits ? 2 : 1;
+    uint32_t *k2 = reinterpret_cast<uint32_t *>(b->k2);
+    system_key_kernel<<<g, 256, 0, s>>>(n, b->perm1, b->sys_of, k2);
+    launch_radix_passes(b->hist, s, k2, b->perm1, n, 0, sys_passes, reinterpret_cast<uint32_t *>(b->k2buf), b->perm,
+                        reinterpret_cast<uint32_t *>(b->kbuf), b->vbuf, nullptr);
     // 3. keys in (system, key) order and the inverse permutation
     gather_sorted_kernel<<<g, 256, 0, s>>>(n, b->perm, b->keys, b->skeys, b->pos);
 }
