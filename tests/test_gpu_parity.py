@@ -164,6 +164,28 @@ def test_pipeline_matches_oracle(oracle, kind, n):
     check_forces(bh, t, oracle, float(np.float32(np.float32(params.softening) ** 2)))
 
 
+def test_centre_of_mass_twice_resums_every_cell():
+    """A second centreOfMass on the same tree writes every cell again, subtree
+    cells included (their job counter is set back after each summary): the
+    cells' masses and centres, zeroed in between, come back bit for bit."""
+    _torch()
+    n = 200_000
+    print(f"case centre of mass twice binary_disk n={n}", flush=True)
+    bh = run_to_forces(*make_case("binary_disk", n))
+    a, kw = bh.arr, bh.kw
+    index = int(a["index"].item())
+    assert index - n > 1000  # subtree cells as well as top cells
+    first = {k: a[k][n:index].clone() for k in ("x", "y", "z", "mass")}
+    m64 = bh.ctx.copy_buffer(nns_amd.BUF_CELL_MASS64, index, np.float64)
+    for k in first:
+        a[k][n:index] = 0.0
+    kw.centreOfMass(a["x"], a["y"], a["z"], a["mass"], a["index"], n)
+    bh.ctx.check()
+    for k, v in first.items():
+        assert bool((a[k][n:index] == v).all()), k
+    assert np.array_equal(bh.ctx.copy_buffer(nns_amd.BUF_CELL_MASS64, index, np.float64), m64)
+
+
 @pytest.mark.parametrize("n,m", [(1000, 14000), (4097, 2 * 4097 + 12000), (70000, 70001)])
 def test_reset_arrays_post_state(n, m):
     """nbx_reset_arrays leaves exactly what resetArraysKernel (Kernels.cu:21-64)
