@@ -21,7 +21,8 @@
 // than the walk).  They are appended to a log {32-bit node id, float4}: each
 // wave takes chunks of kChunk records with one atomicAdd and seals the unused
 // tail of a chunk with a sentinel node id.  The log is radix-sorted by node id
-// (stable, 3 passes for up to 2^24 - 1 nodes) and summed per node in FP64, in
+// (stable; 3 passes while the ids -- sorted positions of bodies, n + the first
+// frame record of a cell -- stay below 2^24 - 1) and summed per node in FP64, in
 // log order, over tiles of 64 sorted records (one per lane); only the node runs
 // that cross a tile take an atomic.  The log order is that of the chunk grants,
 // so a node's FP64 total can differ from run to run in its last bits.  A wave
@@ -198,11 +199,11 @@ __global__ void __launch_bounds__(64 * W) vjp_kernel(
         const uint4 e = s[sp];
         const int tag = __builtin_amdgcn_readfirstlane((int)e.x);
         const int nch = tag & 15;
-        const int blk = tag >> 4;
+        const int off = tag >> 4;  // the block's first record
         const uint64_t mask = ((uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)e.w) << 32) |
                               (uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)e.z);
-        const float4 *bq = fnode + 8 * (int64_t)blk;
-        const int2 *bd = fmeta + 8 * (int64_t)blk;
+        const float4 *bq = fnode + off;
+        const int2 *bd = fmeta + off;
         float4 q[8];
         int2 d[8];
 #pragma unroll
@@ -328,6 +329,8 @@ __global__ void __launch_bounds__(256) log_reduce_kernel(const uint32_t *__restr
 
 __global__ void __launch_bounds__(256) adj_cells_kernel(int n, const int *__restrict__ index,
                                                         const int32_t *__restrict__ pblk,
+                                                        const uint32_t *__restrict__ boff,
+                                                        const uint32_t *__restrict__ bsum,
                                                         const double *__restrict__ tgt,
                                                         const double *__restrict__ mass64,
                                                         const double *__restrict__ mom64, double *__restrict__ ab,
@@ -338,7 +341,7 @@ __global__ void __launch_bounds__(256) adj_cells_kernel(int n, const int *__rest
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nblk; b += blockDim.x * gridDim.x) {
         double a0 = 0.0, a1 = 0.0, a2 = 0.0, bb = 0.0;
         for (int c = b; c >= 0; c = pblk[c]) {
-            const double *t = tgt + 4 * ((int64_t)n + c);
+            const double *t = tgt + 4 * ((int64_t)n + frame_off(boff, bsum, c));  // the block's node id
             double M, X, Y, Z;
             if (c == 0) {
                 M = st->root64[0]; X = st->root64[1]; Y = st->root64[2]; Z = st->root64[3];
@@ -384,12 +387,13 @@ __global__ void __launch_bounds__(256) adj_bodies_kernel(int n, const int32_t *_
     }
 }
 
-// Radix passes over the node ids: ids are < m_max + 1 and the sentinel,
-// 2^(kRadixBits passes) - 1, must exceed them.
-static int sort_passes(int64_t m_max)
+// Radix passes over the node ids: a body's id is its sorted position, a cell's
+// n + the first record of its frame block, so ids are below n_max + m_max and
+// the sentinel, 2^(kRadixBits passes) - 1, must exceed them.
+static int sort_passes(int64_t id_max)
 {
     int p = 1;
-    while ((int64_t(1) << (kRadixBits * p)) < m_max + 2) ++p;
+    while ((int64_t(1) << (kRadixBits * p)) < id_max + 2) ++p;
     return p;
 }
 
@@ -405,12 +409,12 @@ template <class T> static bool dmalloc(T **p, int64_t count)
 
 bool alloc_vjp_scratch(nbx_context *c)
 {
-    const int64_t cnt = 4 * ((int64_t)c->m_max + 1);
+    const int64_t cnt = 4 * ((int64_t)c->n_max + c->m_max + 8);  // node ids: bodies, then n + a record
     const int64_t ngroups = ((int64_t)c->n_max + 63) / 64;
     // about 60 records per body at theta = 0.5; a full log falls back to atomics
     int64_t cap = 64 * (int64_t)c->n_max + (int64_t)kChunk * (ngroups + 1);
     cap = std::min<int64_t>(cap, INT32_MAX - kSortTile);
-    const bool even = sort_passes(c->m_max) % 2 == 0;
+    const bool even = sort_passes((int64_t)c->n_max + c->m_max) % 2 == 0;
     const int64_t tiles = (cap + kSortTile - 1) / kSortTile;
     bool ok = dmalloc(&c->adj_tgt, cnt) && dmalloc(&c->adj_ab, cnt) && dmalloc(&c->adj_log_id, cap) &&
               dmalloc(&c->adj_log_val, cap) && dmalloc(&c->adj_sk, cap) && dmalloc(&c->adj_sv, cap) &&
@@ -444,10 +448,10 @@ void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy,
     const int n = c->n;
     if (n <= 0) return;
     if (frame_layout_stale(c)) launch_frame_layout(c, s);
-    const int64_t ntgt = 4 * ((int64_t)n + (c->m - n + 1));  // bodies, then blocks
+    const int64_t ntgt = 4 * ((int64_t)n + c->m + 8);  // bodies, then cells by their block's first record
     clear_f64_kernel<<<(int)std::min<int64_t>((ntgt + 255) / 256, 65536), 256, 0, s>>>(c->adj_tgt, ntgt);
     vjp_reset_kernel<<<1, 1, 0, s>>>(c->st);
-    const int passes = sort_passes(c->m_max);
+    const int passes = sort_passes((int64_t)c->n_max + c->m_max);
     const uint32_t sentinel = (uint32_t)((uint64_t(1) << (kRadixBits * passes)) - 1);
     constexpr int W = 2;
     const int ngroups = (n + 63) / 64;
@@ -465,7 +469,7 @@ void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy,
         c->adj_sk, c->adj_sv, c->adj_log_val, c->st, sentinel, c->adj_tgt);
     const int64_t nblk = c->m - n + 1;
     adj_cells_kernel<<<(int)std::min<int64_t>((nblk + 255) / 256, 65536), 256, 0, s>>>(
-        n, c->index, c->pblk, c->adj_tgt, c->mass64, c->mom64, c->adj_ab, c->st);
+        n, c->index, c->pblk, c->boff, c->bsum, c->adj_tgt, c->mass64, c->mom64, c->adj_ab, c->st);
     adj_bodies_kernel<<<(int)std::min<int64_t>((n + 255) / 256, 65536), 256, 0, s>>>(
         n, c->perm, c->bpar, c->nodes4, c->adj_tgt, c->adj_ab, gx, gy, gz, gm, c->st);
 }

@@ -4,12 +4,14 @@
 // per lane, and walks the tree depth first.  The unit of the walk is a frame:
 // an opened cell together with the mask of the lanes that opened it.  A frame
 // names a block of the frame layout (the cell's non-empty children, compacted,
-// with chains of one-child cells collapsed to their bottom cell), and the
-// whole block -- 8 float4 (x, y, z, m) and 8 int2 descriptors, 192 bytes --
-// comes in through the scalar cache with three s_load_dwordx16 once per frame.
-// The children are then stepped through with the data already in SGPRs, so
-// the only dependent memory access of a frame is that one load; frames live in
-// an LDS stack of {tag = block << 4 | children, 0, lane mask low, lane mask high}.
+// with chains of one-child cells collapsed to their bottom cell), packed: the
+// next block's records follow the last child.  Eight records from the block's
+// first -- 8 float4 (x, y, z, m) and 8 int2 descriptors, 192 bytes -- come in
+// through the scalar cache with three s_load_dwordx16 once per frame, and only
+// the block's own children are stepped through, with their data already in
+// SGPRs, so the only dependent memory access of a frame is that one load;
+// frames live in an LDS stack of {tag = first record << 4 | children, 0, lane
+// mask low, lane mask high}.
 //
 // Each lane applies its own opening test to a cell child (l/D < theta written
 // as crit_L + eps^2 < |d|^2 + eps^2 with crit_L = (l_L/theta)^2, the reference
@@ -149,7 +151,7 @@ static __device__ __forceinline__ void walk_group(int g, int lane, uint4 *s, con
         const uint4 e = s[sp];
         const int tag = __builtin_amdgcn_readfirstlane((int)e.x);
         const int nch = tag & 15;
-        const int blk = tag >> 4;
+        const int off = tag >> 4;  // the block's first record
         const uint64_t mask = ((uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)e.w) << 32) |
                               (uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)e.z);
         work += nch;
@@ -157,8 +159,8 @@ static __device__ __forceinline__ void walk_group(int g, int lane, uint4 *s, con
             nsteps += nch;
             nframes += 1;
         }
-        const float4 *bq = fnode + 8 * (int64_t)blk;
-        const int2 *bd = fmeta + 8 * (int64_t)blk;
+        const float4 *bq = fnode + off;
+        const int2 *bd = fmeta + off;
         float4 q[8];
         int2 d[8];
 #pragma unroll

@@ -928,10 +928,11 @@ __global__ void __launch_bounds__(256) sorted_index_kernel(int n, const int32_t 
 }
 
 // Frame blocks.  Block b lists the non-empty children of one cell, compacted in
-// slot order: fnode[8b + k] = (x, y, z, m) of the child and fmeta[8b + k] its
+// slot order, as the records [off(b), off(b) + k_b) of the packed arrays:
+// fnode[off(b) + k] = (x, y, z, m) of the child and fmeta[off(b) + k] its
 // descriptor {tag, crit}:
 //   body:  tag = sorted position << 4,        crit = -1 (always accepted);
-//   cell:  tag = block << 4 | children (1-8), crit = crit_L + eps^2,
+//   cell:  tag = off(block) << 4 | children (1-8), crit = crit_L + eps^2,
 // crit_L = (l_L/theta)^2 computed as in the opening test (crit_0 rounded once
 // from FP64, scaled by 4^-L exactly) and eps^2 added once in FP32, so the walk
 // compares crit < |d|^2 + eps^2 -- the value its interaction needs anyway.
@@ -946,10 +947,55 @@ __global__ void __launch_bounds__(256) sorted_index_kernel(int n, const int32_t 
 // froot4 / frootm describe each root as a child of a virtual parent.  The
 // parent blocks of every block and of every body (by sorted position) go to
 // pblk / bpar for the adjoint's top-down pass.
+// Blocks follow in id order with no gaps between them, so off(b) is the number
+// of children of the blocks before b: the packed records take 24 bytes per node
+// (fnode and fmeta) where blocks of eight slots took 192 bytes per cell, and a
+// frame's three block loads come from fewer distinct lines of the L2.
 template <bool BATCH>
 __device__ inline int block_of(int c, int n)
 {
     return BATCH ? c - n : (c == 0 ? 0 : c - n + 1);
+}
+
+// Children per block, summed within each run of kOffRun blocks: boff[b] = the
+// children of the blocks of b's run before b, bsum[run] = the run's total (the
+// runs are then scanned in place).  One workgroup per run.
+template <bool BATCH>
+__global__ void __launch_bounds__(256) frame_count_kernel(int n, const int *__restrict__ index,
+                                                          const int *__restrict__ child, uint32_t *__restrict__ boff,
+                                                          uint32_t *__restrict__ bsum, const DeviceState *st)
+{
+    __shared__ uint32_t sh[4];
+    static_assert(kOffRun == 4 * 256, "four blocks per thread");
+    if (latched(st)) return;
+    const int64_t items = BATCH ? (int64_t)*index - n : (int64_t)*index - n + 1;
+    const int64_t i0 = (int64_t)blockIdx.x * kOffRun + 4 * threadIdx.x;
+    if ((int64_t)blockIdx.x * kOffRun >= items) {  // uniform: a run past the last block
+        if (threadIdx.x == 0) bsum[blockIdx.x] = 0u;
+        return;
+    }
+    uint32_t k[4], s = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int64_t item = i0 + u;
+        k[u] = 0u;
+        if (item < items) {
+            const int node = BATCH ? n + (int)item : (item == 0 ? 0 : n + (int)item - 1);
+            const int4 *p = reinterpret_cast<const int4 *>(child + 8 * (int64_t)node);
+            const int4 a = p[0], b = p[1];
+            k[u] = (a.x != -1) + (a.y != -1) + (a.z != -1) + (a.w != -1) + (b.x != -1) + (b.y != -1) + (b.z != -1) +
+                   (b.w != -1);
+        }
+        s += k[u];
+    }
+    uint32_t tot;
+    uint32_t ex = block_exclusive_scan(s, sh, &tot);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (i0 + u < items) boff[i0 + u] = ex;
+        ex += k[u];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
 // Opening threshold of a level-L cell as the walk compares it: crit_L + eps^2,
@@ -966,7 +1012,7 @@ __device__ inline float crit_of(double span, float theta, float eps2, int lev)
 template <bool BATCH>
 __device__ inline int2 chain_bottom(int c, int n, const int *__restrict__ child,
                                     const int32_t *__restrict__ cell_level, bool root, double span, float theta,
-                                    float eps2)
+                                    float eps2, const uint32_t *__restrict__ boff, const uint32_t *__restrict__ bsum)
 {
     for (;;) {
         const int4 *p = reinterpret_cast<const int4 *>(child + 8 * (int64_t)c);
@@ -981,7 +1027,8 @@ __device__ inline int2 chain_bottom(int c, int n, const int *__restrict__ child,
             continue;
         }
         const int lev = root ? 0 : cell_level[c];
-        return make_int2((block_of<BATCH>(c, n) << 4) | cnt, __float_as_int(crit_of(span, theta, eps2, lev)));
+        const int off = frame_off(boff, bsum, block_of<BATCH>(c, n));
+        return make_int2((off << 4) | cnt, __float_as_int(crit_of(span, theta, eps2, lev)));
     }
 }
 
@@ -990,7 +1037,8 @@ __device__ inline int2 chain_bottom(int c, int n, const int *__restrict__ child,
 template <bool BATCH>
 __device__ inline int2 chain_bottom_from(int c, int4 a, int4 b, int lev, int n, const int *__restrict__ child,
                                          const int32_t *__restrict__ cell_level, double span, float theta,
-                                         float eps2)
+                                         float eps2, const uint32_t *__restrict__ boff,
+                                         const uint32_t *__restrict__ bsum)
 {
     for (;;) {
         const int v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
@@ -1005,7 +1053,8 @@ __device__ inline int2 chain_bottom_from(int c, int4 a, int4 b, int lev, int n, 
             lev = cell_level[c];
             continue;
         }
-        return make_int2((block_of<BATCH>(c, n) << 4) | cnt, __float_as_int(crit_of(span, theta, eps2, lev)));
+        const int off = frame_off(boff, bsum, block_of<BATCH>(c, n));
+        return make_int2((off << 4) | cnt, __float_as_int(crit_of(span, theta, eps2, lev)));
     }
 }
 
@@ -1021,8 +1070,9 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
     int n, int nroot, const int *__restrict__ index, const int *__restrict__ child, const int32_t *__restrict__ pos,
     const int32_t *__restrict__ cell_level, const int *__restrict__ count, const int *__restrict__ start,
     const float4 *__restrict__ nodes4, float4 *__restrict__ fnode, int2 *__restrict__ fmeta,
-    float4 *__restrict__ froot4, int2 *__restrict__ frootm, int32_t *__restrict__ pblk, int32_t *__restrict__ bpar,
-    float theta, float eps2, SysInfo si, const DeviceState *st)
+    const uint32_t *__restrict__ boff, const uint32_t *__restrict__ bsum, float4 *__restrict__ froot4,
+    int2 *__restrict__ frootm, int32_t *__restrict__ pblk, int32_t *__restrict__ bpar, float theta, float eps2,
+    SysInfo si, const DeviceState *st)
 {
     if (latched(st)) return;
     const int ncell = *index - n;
@@ -1047,6 +1097,7 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
         const int ccnt = count[cq];
         const float4 cnode = nodes4[cq];
         const int nstart = k >= total || (!BATCH && item == 0) ? 0 : start[node];  // the single root is node 0
+        const int boff0 = k < total ? frame_off(boff, bsum, (int)item) : 0;  // the block's first record
         const uint64_t occ = __ballot(ch != -1);
         const unsigned seg = (unsigned)(occ >> (lane & ~7)) & 0xffu;
         const int rank = __popc(seg & ((1u << s) - 1u));
@@ -1074,11 +1125,11 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
         }
         if (k < total && item < nroot && s == 0) {
             froot4[item] = BATCH ? nodes4[node] : st->root4;
-            frootm[item] = chain_bottom<BATCH>(node, n, child, cell_level, true, span, th, eps2);
+            frootm[item] = chain_bottom<BATCH>(node, n, child, cell_level, true, span, th, eps2, boff, bsum);
             pblk[item] = -1;
         }
         if (ch == -1) continue;
-        const int64_t slot = 8 * item + rank;
+        const int64_t slot = (int64_t)boff0 + rank;
         if (ch < n) {
             const int q = nstart + off;
             const int ps = q >= 0 && q < n ? q : pos[ch];
@@ -1087,7 +1138,7 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
             bpar[ps] = (int)item;
         } else {
             fnode[slot] = cnode;
-            fmeta[slot] = chain_bottom_from<BATCH>(ch, ca, cb, clev, n, child, cell_level, span, th, eps2);
+            fmeta[slot] = chain_bottom_from<BATCH>(ch, ca, cb, clev, n, child, cell_level, span, th, eps2, boff, bsum);
             pblk[block_of<BATCH>(ch, n)] = (int)item;
         }
     }
@@ -1222,9 +1273,12 @@ void launch_layout(nbx_context *c, hipStream_t s, int *sorted)
 void launch_frame_layout(nbx_context *c, hipStream_t s)
 {
     SysInfo si{};
+    const int runs = (int)(((int64_t)c->m - c->n + 1 + kOffRun - 1) / kOffRun);
+    frame_count_kernel<false><<<runs, 256, 0, s>>>(c->n, c->index, c->child, c->boff, c->bsum, c->st);
+    launch_exclusive_scan_u32(c->bsum, runs, c->ts.scan_tmp, s);
     frame_layout_kernel<false><<<capped_grid(8 * (int64_t)(c->m - c->n + 1)), 256, 0, s>>>(
-        c->n, 1, c->index, c->child, c->pos, c->cell_level, c->count, c->start, c->nodes4, c->fnode, c->fmeta, c->froot4,
-        c->frootm, c->pblk, c->bpar, c->p.theta, layout_eps2(c->p), si, c->st);
+        c->n, 1, c->index, c->child, c->pos, c->cell_level, c->count, c->start, c->nodes4, c->fnode, c->fmeta, c->boff,
+        c->bsum, c->froot4, c->frootm, c->pblk, c->bpar, c->p.theta, layout_eps2(c->p), si, c->st);
     c->layout_theta = c->p.theta;
     c->layout_eps2 = layout_eps2(c->p);
 }
@@ -1249,9 +1303,12 @@ void launch_batch_layout(nbx_batch *b, hipStream_t s)
 {
     sorted_index_kernel<<<capped_grid(b->N), 256, 0, s>>>(b->N, b->perm, b->sorted, b->st);
     SysInfo si{b->start, b->perm, b->sys_of, b->theta, b->ext};
+    const int runs = (int)(((int64_t)b->m - b->N + kOffRun - 1) / kOffRun);
+    frame_count_kernel<true><<<runs, 256, 0, s>>>(b->N, b->index, b->child, b->boff, b->bsum, b->st);
+    launch_exclusive_scan_u32(b->bsum, runs, b->ts.scan_tmp, s);
     frame_layout_kernel<true><<<capped_grid(8 * (int64_t)(b->m - b->N)), 256, 0, s>>>(
-        b->N, b->S, b->index, b->child, b->pos, b->cell_level, b->count, b->start, b->nodes4, b->fnode, b->fmeta, b->froot4,
-        b->frootm, b->pblk, b->bpar, 0.f, layout_eps2(b->p), si, b->st);
+        b->N, b->S, b->index, b->child, b->pos, b->cell_level, b->count, b->start, b->nodes4, b->fnode, b->fmeta, b->boff,
+        b->bsum, b->froot4, b->frootm, b->pblk, b->bpar, 0.f, layout_eps2(b->p), si, b->st);
     int sys_bits = 0;
     while ((1 << sys_bits) < b->S) ++sys_bits;
     group_keys_kernel<uint64_t><<<capped_grid(b->N), 256, 0, s>>>(b->N, b->skeys, b->perm, b->sys_of, b->gkey, b->st);

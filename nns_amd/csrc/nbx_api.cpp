@@ -133,7 +133,8 @@ int nbx_create(const nbx_params *p, int n_max, int m_max, int device, nbx_contex
               dalloc(&c->hist, (size_t)c->hist_tiles * nbx::kRadix + nbx::kRadix) == hipSuccess &&
               dalloc(&c->cell_level, m) == hipSuccess && dalloc(&c->mass64, m) == hipSuccess &&
               dalloc(&c->mom64, 3 * m) == hipSuccess && dalloc(&c->nodes4, m) == hipSuccess &&
-              dalloc(&c->fnode, 8 * m) == hipSuccess && dalloc(&c->fmeta, 8 * m) == hipSuccess &&
+              dalloc(&c->fnode, m + 8) == hipSuccess && dalloc(&c->fmeta, m + 8) == hipSuccess &&
+              dalloc(&c->boff, m + 1) == hipSuccess && dalloc(&c->bsum, m / nbx::kOffRun + 2) == hipSuccess &&
               dalloc(&c->froot4, 1) == hipSuccess && dalloc(&c->frootm, 1) == hipSuccess &&
               dalloc(&c->pblk, m) == hipSuccess && dalloc(&c->bpar, n) == hipSuccess && dalloc(&c->counts, n) == hipSuccess &&
               dalloc(&c->acc_scratch, 3 * n) == hipSuccess && dalloc(&c->bodies4, n) == hipSuccess &&
@@ -161,7 +162,7 @@ int nbx_destroy(nbx_context *c)
     hipSetDevice(c->device);
     hipDeviceSynchronize();
     void *bufs[] = {c->st, c->bbox_part, c->keys, c->skeys, c->kbuf, c->perm, c->vbuf, c->pos, c->hist,
-                    c->cell_level, c->mass64, c->mom64, c->nodes4, c->fnode, c->fmeta, c->froot4, c->frootm, c->pblk, c->bpar, c->counts,
+                    c->cell_level, c->mass64, c->mom64, c->nodes4, c->fnode, c->fmeta, c->boff, c->bsum, c->froot4, c->frootm, c->pblk, c->bpar, c->counts,
                     c->acc_scratch, c->star_part, c->walk, c->gcost, c->bodies4, c->gkey, c->gkey2,
                     c->gorder};
     for (void *b : bufs)

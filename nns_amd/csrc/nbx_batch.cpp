@@ -132,7 +132,8 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
               dalloc(&b->child, 8 * M) && dalloc(&b->index, 1) && dalloc(&b->count, M) && dalloc(&b->start, M) &&
               dalloc(&b->cell_level, M) &&
               dalloc(&b->mass64, M) && dalloc(&b->mom64, 3 * M) && dalloc(&b->nodes4, M) &&
-              dalloc(&b->fnode, 8 * M) && dalloc(&b->fmeta, 8 * M) && dalloc(&b->froot4, S) &&
+              dalloc(&b->fnode, M + 8) && dalloc(&b->fmeta, M + 8) && dalloc(&b->boff, M) &&
+              dalloc(&b->bsum, M / nbx::kOffRun + 2) && dalloc(&b->froot4, S) &&
               dalloc(&b->frootm, S) && dalloc(&b->pblk, M) && dalloc(&b->bpar, N) && dalloc(&b->counts, N) && dalloc(&b->acc_scratch, 3 * N) && dalloc(&b->bodies4, N) &&
               dalloc(&b->walk, 2 * groups.size()) && dalloc(&b->gcost, groups.size()) &&
               dalloc(&b->gkey, N) && dalloc(&b->gkey2, N) && dalloc(&b->gorder, N) &&
@@ -166,7 +167,7 @@ int nbx_batch_destroy(nbx_batch *b)
     void *bufs[] = {b->st, b->off, b->sys_of, b->theta, b->groups, b->ext, b->mm, b->keys, b->skeys, b->kbuf,
                     b->k2, b->k2buf, b->perm, b->perm1, b->vbuf, b->pos, b->sorted, b->hist, b->child, b->index,
                     b->count, b->start, b->cell_level, b->mass64, b->mom64, b->nodes4,
-                    b->fnode, b->fmeta, b->froot4, b->frootm, b->pblk, b->bpar, b->counts, b->acc_scratch, b->samp_off, b->bodies4, b->gkey, b->gkey2, b->gorder, b->walk, b->gcost,
+                    b->fnode, b->fmeta, b->boff, b->bsum, b->froot4, b->frootm, b->pblk, b->bpar, b->counts, b->acc_scratch, b->samp_off, b->bodies4, b->gkey, b->gkey2, b->gorder, b->walk, b->gcost,
                     b->bchunk, b->bchunk_off, b->bpart};
     for (void *q : bufs)
         if (q) hipFree(q);

@@ -110,6 +110,13 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh, uint32
     return pre + x - v;
 }
 
+// First record of frame block b in the packed layout: the sum within its run of
+// kOffRun blocks plus the records of the runs before it.
+__device__ inline int frame_off(const uint32_t *__restrict__ boff, const uint32_t *__restrict__ bsum, int b)
+{
+    return (int)(boff[b] + bsum[b / kOffRun]);
+}
+
 // ---------------------------------------------------------------- tree
 static __device__ inline int digit_at(uint64_t key, int level) { return (int)((key >> (60 - 3 * level)) & 7u); }
 

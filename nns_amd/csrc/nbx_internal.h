@@ -46,9 +46,13 @@ constexpr int kLevels = kKeyLevels + 1;  // levels 0..21 per sample in the top t
 // (each system's samples rounded up), is 47,662 samples = 48,805,888 bodies.
 constexpr int kMaxSamples = (1 << 20) / kLevels;
 constexpr int kMaxBodies = kMaxSamples * kSubtreeMax;
-// A frame descriptor's tag holds a block index (or a sorted position) times 16
-// in an int: nodes stay below 2^27.
+// A frame descriptor's tag holds a record offset (or a sorted position) times 16
+// in an int: nodes, and so records, stay below 2^27.
 constexpr int kMaxNodes = 1 << 27;
+// Frame blocks are packed: block b's children are records [off(b), off(b) + k_b)
+// of fnode / fmeta.  off(b) is summed over runs of kOffRun blocks, then over the
+// runs (frame_count_kernel, then a scan of the run totals).
+constexpr int kOffRun = 1024;
 
 // Arguments of a force launch that a group needs only when its walk starts or
 // ends.  A setup kernel stores them in device memory before the launch and the
@@ -136,14 +140,16 @@ struct nbx_context {
     int32_t *cell_level = nullptr;
     double *mass64 = nullptr, *mom64 = nullptr;
     float4 *nodes4 = nullptr;
-    float4 *fnode = nullptr;     // [8 * m_max] frame blocks of the force walk: child (x, y, z, m)
-    int2 *fmeta = nullptr;       // [8 * m_max] child descriptors
+    float4 *fnode = nullptr;     // [m_max + 8] frame records of the force walk: child (x, y, z, m)
+    int2 *fmeta = nullptr;       // [m_max + 8] child descriptors
+    uint32_t *boff = nullptr;    // [m_max + 1] record offset of each block within its run of kOffRun
+    uint32_t *bsum = nullptr;    // [m_max / kOffRun + 1] records before each run
     float4 *froot4 = nullptr;    // root as a child of a virtual parent
     int2 *frootm = nullptr;
     int32_t *pblk = nullptr;     // [m_max] parent block of each block (-1: root)
     int32_t *bpar = nullptr;     // [n_max] parent block of each body, by sorted position
     // adjoint scratch (allocated by the first nbx_force_vjp)
-    double *adj_tgt = nullptr;   // [4 (m_max + 1)] gradient sums of bodies (sorted), then blocks
+    double *adj_tgt = nullptr;   // [4 (n_max + m_max + 1)] gradient sums of bodies (sorted), then records
     double *adj_ab = nullptr;    // [4 (m_max + 1)] per block: sum over ancestors of G/M and GM - G.X/M
     // node-term log: one record {node, float4} per child step with accepting lanes
     int64_t adj_cap = 0;         // records
@@ -208,8 +214,10 @@ struct nbx_batch {
     int32_t *cell_level = nullptr;
     double *mass64 = nullptr, *mom64 = nullptr;
     float4 *nodes4 = nullptr;
-    float4 *fnode = nullptr;     // [8 * M] frame blocks
+    float4 *fnode = nullptr;     // [M + 8] frame records
     int2 *fmeta = nullptr;
+    uint32_t *boff = nullptr;    // [M] record offset of each block within its run of kOffRun
+    uint32_t *bsum = nullptr;    // [M / kOffRun + 1] records before each run
     float4 *froot4 = nullptr;    // [S] roots
     int2 *frootm = nullptr;
     int32_t *pblk = nullptr, *bpar = nullptr;
