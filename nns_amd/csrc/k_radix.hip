@@ -316,6 +316,38 @@ void launch_batch_sort(nbx_batch *b, hipStream_t s)
     gather_sorted_kernel<<<g, 256, 0, s>>>(n, b->perm, b->keys, b->skeys, b->pos);
 }
 
+// Exclusive scan of `total` uint32 in place by one workgroup of 1024 threads,
+// 16 entries per thread per pass, the passes chained by a carry: one launch,
+// for arrays of a few thousand entries.
+__global__ void __launch_bounds__(1024) scan_one_block_kernel(uint32_t *__restrict__ data, int total)
+{
+    __shared__ uint32_t sh[16];
+    constexpr int V = 16;
+    uint32_t carry = 0;
+    for (int t0 = 0; t0 < total; t0 += V * (int)blockDim.x) {
+        const int i0 = t0 + V * (int)threadIdx.x;
+        uint32_t v[V], sum = 0;
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            v[u] = i0 + u < total ? data[i0 + u] : 0u;
+            sum += v[u];
+        }
+        uint32_t tot;
+        uint32_t ex = block_exclusive_scan(sum, sh, &tot) + carry;
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            if (i0 + u < total) data[i0 + u] = ex;
+            ex += v[u];
+        }
+        carry += tot;
+    }
+}
+
+void launch_scan_one_block(uint32_t *data, int total, hipStream_t s)
+{
+    if (total > 0) scan_one_block_kernel<<<1, 1024, 0, s>>>(data, total);
+}
+
 void launch_exclusive_scan_u32(uint32_t *data, int total, uint32_t *sums, hipStream_t s)
 {
     if (total <= 0) return;

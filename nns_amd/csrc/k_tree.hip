@@ -959,40 +959,47 @@ __device__ inline int block_of(int c, int n)
 
 // Children per block, summed within each run of kOffRun blocks: boff[b] = the
 // children of the blocks of b's run before b, bsum[run] = the run's total (the
-// runs are then scanned in place).  One workgroup per run.
+// runs are then scanned in place).  One workgroup per run.  The slots are read
+// as 16-byte halves of the blocks' rows, consecutive threads on consecutive
+// halves, and the two halves of a block meet by a lane exchange.
 template <bool BATCH>
 __global__ void __launch_bounds__(256) frame_count_kernel(int n, const int *__restrict__ index,
                                                           const int *__restrict__ child, uint32_t *__restrict__ boff,
                                                           uint32_t *__restrict__ bsum, const DeviceState *st)
 {
+    __shared__ uint32_t cnt[kOffRun];
     __shared__ uint32_t sh[4];
     static_assert(kOffRun == 4 * 256, "four blocks per thread");
     if (latched(st)) return;
     const int64_t items = BATCH ? (int64_t)*index - n : (int64_t)*index - n + 1;
-    const int64_t i0 = (int64_t)blockIdx.x * kOffRun + 4 * threadIdx.x;
-    if ((int64_t)blockIdx.x * kOffRun >= items) {  // uniform: a run past the last block
+    const int64_t r0 = (int64_t)blockIdx.x * kOffRun;
+    if (r0 >= items) {  // uniform: a run past the last block
         if (threadIdx.x == 0) bsum[blockIdx.x] = 0u;
         return;
     }
-    uint32_t k[4], s = 0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int64_t item = i0 + u;
-        k[u] = 0u;
+    for (int u = 0; u < 8; ++u) {
+        const int h = (int)threadIdx.x + 256 * u;  // half h & 1 of the run's block h >> 1
+        const int64_t item = r0 + (h >> 1);
+        int k = 0;
         if (item < items) {
             const int node = BATCH ? n + (int)item : (item == 0 ? 0 : n + (int)item - 1);
-            const int4 *p = reinterpret_cast<const int4 *>(child + 8 * (int64_t)node);
-            const int4 a = p[0], b = p[1];
-            k[u] = (a.x != -1) + (a.y != -1) + (a.z != -1) + (a.w != -1) + (b.x != -1) + (b.y != -1) + (b.z != -1) +
-                   (b.w != -1);
+            const int4 q = reinterpret_cast<const int4 *>(child + 8 * (int64_t)node)[h & 1];
+            k = (q.x != -1) + (q.y != -1) + (q.z != -1) + (q.w != -1);
         }
-        s += k[u];
+        k += __shfl_xor(k, 1);
+        if ((h & 1) == 0) cnt[h >> 1] = (uint32_t)k;
     }
+    __syncthreads();
+    const int j0 = 4 * (int)threadIdx.x;
+    uint32_t k[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) k[u] = cnt[j0 + u];
     uint32_t tot;
-    uint32_t ex = block_exclusive_scan(s, sh, &tot);
+    uint32_t ex = block_exclusive_scan(k[0] + k[1] + k[2] + k[3], sh, &tot);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        if (i0 + u < items) boff[i0 + u] = ex;
+        if (r0 + j0 + u < items) boff[r0 + j0 + u] = ex;
         ex += k[u];
     }
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
@@ -1275,7 +1282,7 @@ void launch_frame_layout(nbx_context *c, hipStream_t s)
     SysInfo si{};
     const int runs = (int)(((int64_t)c->m - c->n + 1 + kOffRun - 1) / kOffRun);
     frame_count_kernel<false><<<runs, 256, 0, s>>>(c->n, c->index, c->child, c->boff, c->bsum, c->st);
-    launch_exclusive_scan_u32(c->bsum, runs, c->ts.scan_tmp, s);
+    launch_scan_one_block(c->bsum, runs, s);
     frame_layout_kernel<false><<<capped_grid(8 * (int64_t)(c->m - c->n + 1)), 256, 0, s>>>(
         c->n, 1, c->index, c->child, c->pos, c->cell_level, c->count, c->start, c->nodes4, c->fnode, c->fmeta, c->boff,
         c->bsum, c->froot4, c->frootm, c->pblk, c->bpar, c->p.theta, layout_eps2(c->p), si, c->st);
@@ -1305,7 +1312,7 @@ void launch_batch_layout(nbx_batch *b, hipStream_t s)
     SysInfo si{b->start, b->perm, b->sys_of, b->theta, b->ext};
     const int runs = (int)(((int64_t)b->m - b->N + kOffRun - 1) / kOffRun);
     frame_count_kernel<true><<<runs, 256, 0, s>>>(b->N, b->index, b->child, b->boff, b->bsum, b->st);
-    launch_exclusive_scan_u32(b->bsum, runs, b->ts.scan_tmp, s);
+    launch_scan_one_block(b->bsum, runs, s);
     frame_layout_kernel<true><<<capped_grid(8 * (int64_t)(b->m - b->N)), 256, 0, s>>>(
         b->N, b->S, b->index, b->child, b->pos, b->cell_level, b->count, b->start, b->nodes4, b->fnode, b->fmeta, b->boff,
         b->bsum, b->froot4, b->frootm, b->pblk, b->bpar, 0.f, layout_eps2(b->p), si, b->st);

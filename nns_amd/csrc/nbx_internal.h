@@ -267,6 +267,8 @@ void launch_radix_sort(nbx_context *c, hipStream_t s, const uint64_t *keys_in, i
 constexpr int kMortonSortBits = 63;
 // exclusive scan of total <= 2^20 values; sums: scratch of 1024 values
 void launch_exclusive_scan_u32(uint32_t *data, int total, uint32_t *sums, hipStream_t s);
+// The same in one launch of one workgroup (short arrays).
+void launch_scan_one_block(uint32_t *data, int total, hipStream_t s);
 // npasses passes of kRadixBits bits from bit first_shift; vals_in == nullptr means 0, 1, 2, ...
 // n_dev (nullable): device-side count; only the first min(n, *n_dev) keys are sorted
 // (grids are sized for n).  tmp_k may alias keys_in when npasses is odd.  K: uint64_t
