@@ -655,13 +655,15 @@ struct CellSum {
 // child is read from the sorted copy at a running cursor; otherwise (first < 0)
 // from its own index, with its sorted position from pos[].
 // The slots go in batches of B (a divisor of 8): per batch (1) the cell
-// children's sums, counts and first bodies, every load issued before the
-// first use (a slot without a cell loads node 0, unused), (2) the body
-// children at the cursor (the cell's first body plus the sizes of the slots
-// before them), (3) the sums in slot order, in FP64.  B = 2 for the subtrees
-// (more would cost occupancy: 4 needs 90 registers against 59), 4 for the
-// top cells (one workgroup).
-template <int B>
+// children's sums, counts and (without a cursor) first bodies, every load
+// issued before the first use (a slot without a cell loads node 0, unused),
+// (2) the body children at the cursor (the cell's first body plus the sizes of
+// the slots before them), (3) the sums in slot order, in FP64.  B = 2 for the
+// subtrees (more would cost occupancy: 4 needs 90 registers against 59), 4 for
+// the top cells.  CURSOR: the build has given every cell its first sorted body
+// (first >= 0), which the summary then neither reads for the children nor
+// stores again.
+template <int B, bool CURSOR>
 __device__ inline CellSum sum_cell(int c, int first, bool bucket, int n, int *child, const int *lslots,
                                    const float4 *bodies4, const float4 *sorted4, const int32_t *pos,
                                    const double *mass64, const double *mom64, const int *count, const int *start)
@@ -693,7 +695,7 @@ __device__ inline CellSum sum_cell(int c, int first, bool bucket, int n, int *ch
             cy[t] = mom64[3 * q + 1];
             cz[t] = mom64[3 * q + 2];
             kc[t] = count[q];
-            kf[t] = start[q];
+            kf[t] = CURSOR ? 0 : start[q];
         }
 #pragma unroll
         for (int t = 0; t < B; ++t) {
@@ -720,14 +722,14 @@ __device__ inline CellSum sum_cell(int c, int first, bool bucket, int n, int *ch
                 r.my = __dadd_rn(r.my, __dmul_rn(mb, (double)b[t].y));
                 r.mz = __dadd_rn(r.mz, __dmul_rn(mb, (double)b[t].z));
                 r.cnt += 1;
-                r.first = min(r.first, bp[t] >= 0 ? bp[t] : pos[ch]);
+                if (!CURSOR) r.first = min(r.first, bp[t] >= 0 ? bp[t] : pos[ch]);
             } else {
                 r.ms = __dadd_rn(r.ms, cm[t]);
                 r.mx = __dadd_rn(r.mx, cx[t]);
                 r.my = __dadd_rn(r.my, cy[t]);
                 r.mz = __dadd_rn(r.mz, cz[t]);
                 r.cnt += kc[t];
-                r.first = min(r.first, kf[t]);
+                if (!CURSOR) r.first = min(r.first, kf[t]);
             }
         }
     }
@@ -749,14 +751,14 @@ struct SumOut {
 
 // lev: the cell's level when the caller has it (else -1); lslots: an LDS copy of
 // its slots (else nullptr).
-// B: slots whose loads go together (sum_cell).
-template <int B>
+// B: slots whose loads go together; CURSOR: o.cursor (sum_cell).
+template <int B, bool CURSOR>
 __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool single_root, int root_first = 0,
                                  int lev = -1, const int *lslots = nullptr)
 {
     const bool bucket = !single_root && (lev >= 0 ? lev : o.cell_level[c]) == kKeyLevels;
-    const int first = !o.cursor ? -1 : single_root ? root_first : o.start[c];
-    const CellSum r = sum_cell<B>(c, first, bucket, o.n, o.child, lslots, o.bodies4, o.nodes4, o.pos, o.mass64,
+    const int first = !CURSOR ? -1 : single_root ? root_first : o.start[c];
+    const CellSum r = sum_cell<B, CURSOR>(c, first, bucket, o.n, o.child, lslots, o.bodies4, o.nodes4, o.pos, o.mass64,
                                   o.mom64, o.count, o.start);
     if (r.cnt > 0 && !(r.ms > 0.0)) atomicOr(&st->err, kErrNonFinite);
     const double gx = r.mx / r.ms, gy = r.my / r.ms, gz = r.mz / r.ms;
@@ -772,7 +774,7 @@ __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool s
         o.cx[c] = fx; o.cy[c] = fy; o.cz[c] = fz; o.mass[c] = fm;
     }
     o.count[c] = r.cnt;
-    o.start[c] = r.first;
+    if (!CURSOR) o.start[c] = r.first;
     o.nodes4[c] = make_float4(fx, fy, fz, fm);
 }
 
@@ -783,6 +785,7 @@ __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool s
 // so a level costs the reads of the children's sums and the stores only.
 constexpr int kSumLdsCells = 512;
 constexpr int kSubSumBatch = 2;
+template <bool CURSOR>
 __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const int4 *__restrict__ subcells,
                                                               DeviceState *st)
 {
@@ -815,7 +818,8 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
         for (int L = sc.w; L >= L0; --L) {
             for (int k = tid; k < ks; k += blockDim.x) {
                 const int lev = lds ? s_lev[k] : o.cell_level[base + k];
-                if (lev == L) summarise<kSubSumBatch>(o, base + k, st, false, 0, lev, lds ? s_slot + 8 * k : nullptr);
+                if (lev == L)
+                    summarise<kSubSumBatch, CURSOR>(o, base + k, st, false, 0, lev, lds ? s_slot + 8 * k : nullptr);
             }
             __syncthreads();
         }
@@ -860,6 +864,7 @@ __device__ inline void top_barrier(DeviceState *st, int phase, bool &stalled)
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
+template <bool CURSOR>
 __global__ void __launch_bounds__(256) top_summary_kernel(SumOut o, Segs g, int nsamples,
                                                           const int32_t *__restrict__ top_list, DeviceState *st)
 {
@@ -884,17 +889,17 @@ __global__ void __launch_bounds__(256) top_summary_kernel(SumOut o, Segs g, int 
         if (cnt <= kTopNarrow) break;
         if (!stop)
             for (int i = blockIdx.x + gridDim.x * tid; i < cnt; i += gridDim.x * blockDim.x)
-                summarise<4>(o, top_list[L * nsamples + i], st, false);
+                summarise<4, CURSOR>(o, top_list[L * nsamples + i], st, false);
         top_barrier(st, ++phase, stalled);
     }
     if (blockIdx.x == 0 && !stop) {
         for (; L >= 1; --L) {
             const int cnt = s_cnt[L];
             if (cnt == 0) continue;
-            for (int i = tid; i < cnt; i += blockDim.x) summarise<4>(o, top_list[L * nsamples + i], st, false);
+            for (int i = tid; i < cnt; i += blockDim.x) summarise<4, CURSOR>(o, top_list[L * nsamples + i], st, false);
             __syncthreads();
         }
-        for (int s = tid; s < g.nseg; s += blockDim.x) summarise<4>(o, seg_root(g, s), st, !g.batch);
+        for (int s = tid; s < g.nseg; s += blockDim.x) summarise<4, CURSOR>(o, seg_root(g, s), st, !g.batch);
     }
     if (tid == 0) {  // the last workgroup out (every other one is past its last wait) resets the barrier
         const int out = __hip_atomic_fetch_add(&st->tdone, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
@@ -1215,8 +1220,13 @@ static void build_tree(Segs g, int nsamples, int base0, int m, TreeScratch &ts, 
 static void summarise_tree(Segs g, int nsamples, TreeScratch &ts, DeviceState *st, hipStream_t s,
                            const SumOut &o)
 {
-    subtree_summary_kernel<<<kPersistentBlocks, 256, 0, s>>>(o, ts.subcells, st);
-    top_summary_kernel<<<kTopBlocks, 256, 0, s>>>(o, g, nsamples, ts.top_list, st);
+    if (o.cursor) {
+        subtree_summary_kernel<true><<<kPersistentBlocks, 256, 0, s>>>(o, ts.subcells, st);
+        top_summary_kernel<true><<<kTopBlocks, 256, 0, s>>>(o, g, nsamples, ts.top_list, st);
+    } else {
+        subtree_summary_kernel<false><<<kPersistentBlocks, 256, 0, s>>>(o, ts.subcells, st);
+        top_summary_kernel<false><<<kTopBlocks, 256, 0, s>>>(o, g, nsamples, ts.top_list, st);
+    }
 }
 
 static inline int samples_of(int n) { return (n + kSubtreeMax - 1) / kSubtreeMax; }
