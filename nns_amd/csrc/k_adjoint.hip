@@ -332,8 +332,7 @@ __global__ void __launch_bounds__(256) adj_cells_kernel(int n, const int *__rest
                                                         const uint32_t *__restrict__ boff,
                                                         const uint32_t *__restrict__ bsum,
                                                         const double *__restrict__ tgt,
-                                                        const double *__restrict__ mass64,
-                                                        const double *__restrict__ mom64, double *__restrict__ ab,
+                                                        const double4 *__restrict__ sum64, double *__restrict__ ab,
                                                         const DeviceState *st)
 {
     if (latched(st)) return;
@@ -347,8 +346,9 @@ __global__ void __launch_bounds__(256) adj_cells_kernel(int n, const int *__rest
                 M = st->root64[0]; X = st->root64[1]; Y = st->root64[2]; Z = st->root64[3];
             } else {
                 const int64_t cell = (int64_t)c + n - 1;
-                M = mass64[cell];
-                X = mom64[3 * cell] / M; Y = mom64[3 * cell + 1] / M; Z = mom64[3 * cell + 2] / M;
+                const double4 cs = sum64[cell];
+                M = cs.x;
+                X = cs.y / M; Y = cs.z / M; Z = cs.w / M;
             }
             if (M > 0.0) {
                 const double g0 = t[0] / M, g1 = t[1] / M, g2 = t[2] / M;
@@ -469,7 +469,7 @@ void launch_vjp(nbx_context *c, hipStream_t s, const float *ux, const float *uy,
         c->adj_sk, c->adj_sv, c->adj_log_val, c->st, sentinel, c->adj_tgt);
     const int64_t nblk = c->m - n + 1;
     adj_cells_kernel<<<(int)std::min<int64_t>((nblk + 255) / 256, 65536), 256, 0, s>>>(
-        n, c->index, c->pblk, c->boff, c->bsum, c->adj_tgt, c->mass64, c->mom64, c->adj_ab, c->st);
+        n, c->index, c->pblk, c->boff, c->bsum, c->adj_tgt, c->sum64, c->adj_ab, c->st);
     adj_bodies_kernel<<<(int)std::min<int64_t>((n + 255) / 256, 65536), 256, 0, s>>>(
         n, c->perm, c->bpar, c->nodes4, c->adj_tgt, c->adj_ab, gx, gy, gz, gm, c->st);
 }

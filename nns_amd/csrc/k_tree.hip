@@ -666,7 +666,7 @@ struct CellSum {
 template <int B, bool CURSOR>
 __device__ inline CellSum sum_cell(int c, int first, bool bucket, int n, int *child, const int *lslots,
                                    const float4 *bodies4, const float4 *sorted4, const int32_t *pos,
-                                   const double *mass64, const double *mom64, const int *count, const int *start)
+                                   const double4 *sum64, const int *count, const int *start)
 {
     int v[8];
     const int *src = lslots ? lslots : child + 8 * (int64_t)c;
@@ -690,10 +690,11 @@ __device__ inline CellSum sum_cell(int c, int first, bool bucket, int n, int *ch
 #pragma unroll
         for (int t = 0; t < B; ++t) {
             const int64_t q = v[s0 + t] >= n ? v[s0 + t] : 0;
-            cm[t] = mass64[q];
-            cx[t] = mom64[3 * q + 0];
-            cy[t] = mom64[3 * q + 1];
-            cz[t] = mom64[3 * q + 2];
+            const double4 cs = sum64[q];  // one 32-byte record: mass and the three moments
+            cm[t] = cs.x;
+            cx[t] = cs.y;
+            cy[t] = cs.z;
+            cz[t] = cs.w;
             kc[t] = count[q];
             kf[t] = CURSOR ? 0 : start[q];
         }
@@ -744,7 +745,7 @@ struct SumOut {
     float *cx, *cy, *cz;     // cell centre of mass (single system; nullptr for the ensemble)
     int *count, *start;      // start: set by the build for every cell when `cursor`
     const int32_t *pos, *cell_level;
-    double *mass64, *mom64;
+    double4 *sum64;          // FP64 cell sums {mass, moments x, y, z} (written)
     float4 *nodes4;          // bodies in key order at [0, n) (read), cells at [n, m) (written)
     int cursor;              // cell starts known before the summary
 };
@@ -758,8 +759,8 @@ __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool s
 {
     const bool bucket = !single_root && (lev >= 0 ? lev : o.cell_level[c]) == kKeyLevels;
     const int first = !CURSOR ? -1 : single_root ? root_first : o.start[c];
-    const CellSum r = sum_cell<B, CURSOR>(c, first, bucket, o.n, o.child, lslots, o.bodies4, o.nodes4, o.pos, o.mass64,
-                                  o.mom64, o.count, o.start);
+    const CellSum r = sum_cell<B, CURSOR>(c, first, bucket, o.n, o.child, lslots, o.bodies4, o.nodes4, o.pos, o.sum64,
+                                  o.count, o.start);
     if (r.cnt > 0 && !(r.ms > 0.0)) atomicOr(&st->err, kErrNonFinite);
     const double gx = r.mx / r.ms, gy = r.my / r.ms, gz = r.mz / r.ms;
     const float fx = (float)gx, fy = (float)gy, fz = (float)gz, fm = (float)r.ms;
@@ -768,8 +769,7 @@ __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool s
         st->root4 = make_float4(fx, fy, fz, fm);
         return;
     }
-    o.mass64[c] = r.ms;
-    o.mom64[3 * (int64_t)c + 0] = r.mx; o.mom64[3 * (int64_t)c + 1] = r.my; o.mom64[3 * (int64_t)c + 2] = r.mz;
+    o.sum64[c] = make_double4(r.ms, r.mx, r.my, r.mz);
     if (o.cx) {
         o.cx[c] = fx; o.cy[c] = fy; o.cz[c] = fz; o.mass[c] = fm;
     }
@@ -1272,7 +1272,7 @@ void launch_summarize(nbx_context *c, hipStream_t s)
 {
     Segs g{nullptr, nullptr, 1, c->n, 0};
     SumOut o{c->n, c->child, c->bodies4, c->mass, c->x, c->y, c->z, c->count, c->start, c->pos,
-             c->cell_level, c->mass64, c->mom64, c->nodes4, c->p.build_locked == 0};
+             c->cell_level, c->sum64, c->nodes4, c->p.build_locked == 0};
     summarise_tree(g, samples_of(c->n), c->ts, c->st, s, o);
 }
 
@@ -1312,7 +1312,7 @@ void launch_batch_summarize(nbx_batch *b, hipStream_t s)
 {
     Segs g{b->off, b->samp_off, b->S, b->N, 1};
     SumOut o{b->N, b->child, b->bodies4, b->mass, nullptr, nullptr, nullptr, b->count, b->start, b->pos,
-             b->cell_level, b->mass64, b->mom64, b->nodes4, b->p.build_locked == 0};
+             b->cell_level, b->sum64, b->nodes4, b->p.build_locked == 0};
     summarise_tree(g, b->ts.samples_max, b->ts, b->st, s, o);
 }
 

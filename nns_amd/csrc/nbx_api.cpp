@@ -19,6 +19,29 @@
 
 using nbx::DeviceState;
 
+namespace nbx {
+
+hipError_t copy_cell_sums(const double4 *sum64, bool com, double *dst, size_t count)
+{
+    const size_t ncell = com ? (count + 2) / 3 : count;
+    std::vector<double4> v(ncell ? ncell : 1);
+    const hipError_t e = hipMemcpy(v.data(), sum64, sizeof(double4) * ncell, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    for (size_t k = 0; k < count; ++k) {
+        const double4 &s = v[com ? k / 3 : k];
+        if (!com) {
+            dst[k] = s.x;
+        } else {
+            const int a = (int)(k % 3);
+            const double mk = a == 0 ? s.y : a == 1 ? s.z : s.w;
+            dst[k] = s.x != 0.0 ? mk / s.x : 0.0;
+        }
+    }
+    return hipSuccess;
+}
+
+}  // namespace nbx
+
 namespace {
 
 thread_local std::string g_create_error;
@@ -131,8 +154,8 @@ int nbx_create(const nbx_params *p, int n_max, int m_max, int device, nbx_contex
               dalloc(&c->kbuf, n) == hipSuccess && dalloc(&c->perm, n) == hipSuccess &&
               dalloc(&c->vbuf, n) == hipSuccess && dalloc(&c->pos, n) == hipSuccess &&
               dalloc(&c->hist, (size_t)c->hist_tiles * nbx::kRadix + nbx::kRadix) == hipSuccess &&
-              dalloc(&c->cell_level, m) == hipSuccess && dalloc(&c->mass64, m) == hipSuccess &&
-              dalloc(&c->mom64, 3 * m) == hipSuccess && dalloc(&c->nodes4, m) == hipSuccess &&
+              dalloc(&c->cell_level, m) == hipSuccess && dalloc(&c->sum64, m) == hipSuccess &&
+              dalloc(&c->nodes4, m) == hipSuccess &&
               dalloc(&c->fnode, m + 8) == hipSuccess && dalloc(&c->fmeta, m + 8) == hipSuccess &&
               dalloc(&c->boff, m + 1) == hipSuccess && dalloc(&c->bsum, m / nbx::kOffRun + 2) == hipSuccess &&
               dalloc(&c->froot4, 1) == hipSuccess && dalloc(&c->frootm, 1) == hipSuccess &&
@@ -162,7 +185,7 @@ int nbx_destroy(nbx_context *c)
     hipSetDevice(c->device);
     hipDeviceSynchronize();
     void *bufs[] = {c->st, c->bbox_part, c->keys, c->skeys, c->kbuf, c->perm, c->vbuf, c->pos, c->hist,
-                    c->cell_level, c->mass64, c->mom64, c->nodes4, c->fnode, c->fmeta, c->boff, c->bsum, c->froot4, c->frootm, c->pblk, c->bpar, c->counts,
+                    c->cell_level, c->sum64, c->nodes4, c->fnode, c->fmeta, c->boff, c->bsum, c->froot4, c->frootm, c->pblk, c->bpar, c->counts,
                     c->acc_scratch, c->star_part, c->walk, c->gcost, c->bodies4, c->gkey, c->gkey2,
                     c->gorder};
     for (void *b : bufs)
@@ -524,29 +547,19 @@ int nbx_copy_buffer(nbx_context *c, int which, void *dst, int64_t count, void *s
     case NBX_BUF_SORTED_KEYS: src = c->skeys; elem = 8; cap = c->n_max; break;
     case NBX_BUF_PERM: src = c->perm; elem = 4; cap = c->n_max; break;
     case NBX_BUF_CELL_LEVEL: src = c->cell_level; elem = 4; cap = c->m_max; break;
-    case NBX_BUF_CELL_MASS64: src = c->mass64; elem = 8; cap = c->m_max; break;
-    case NBX_BUF_CELL_COM64: src = c->mom64; elem = 8; cap = 3 * (size_t)c->m_max; break;
+    case NBX_BUF_CELL_MASS64: elem = 8; cap = c->m_max; break;
+    case NBX_BUF_CELL_COM64: elem = 8; cap = 3 * (size_t)c->m_max; break;
     case NBX_BUF_ROOT: src = c->st->root64; elem = 8; cap = 4; break;
     case NBX_BUF_EXTENT: src = c->st->ext; elem = 8; cap = 4; break;
     default: return fail(c, NBX_ERR_INVALID, "unknown buffer");
     }
     if ((size_t)count > cap) return fail(c, NBX_ERR_INVALID, "count exceeds buffer");
     NBX_HIP(c, hipStreamSynchronize((hipStream_t)stream));
-    NBX_HIP(c, hipMemcpy(dst, src, elem * (size_t)count, hipMemcpyDeviceToHost));
-    if (which == NBX_BUF_CELL_COM64) {
-        // stored as mass moments; convert to centres of mass with the FP64 masses
-        double *d = static_cast<double *>(dst);
-        size_t ncell = (size_t)count / 3;
-        double *ms = new double[ncell ? ncell : 1];
-        hipError_t e = hipMemcpy(ms, c->mass64, 8 * ncell, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) {
-            delete[] ms;
-            return hip_fail(c, e, "hipMemcpy");
-        }
-        for (size_t k = 0; k < ncell; ++k)
-            for (int a = 0; a < 3; ++a) d[3 * k + a] = ms[k] != 0.0 ? d[3 * k + a] / ms[k] : 0.0;
-        delete[] ms;
-    }
+    if (which == NBX_BUF_CELL_MASS64 || which == NBX_BUF_CELL_COM64)
+        NBX_HIP(c, nbx::copy_cell_sums(c->sum64, which == NBX_BUF_CELL_COM64, static_cast<double *>(dst),
+                                       (size_t)count));
+    else
+        NBX_HIP(c, hipMemcpy(dst, src, elem * (size_t)count, hipMemcpyDeviceToHost));
     return NBX_OK;
 }
 

@@ -131,7 +131,7 @@ int nbx_batch_create(const nbx_params *p, int nsys, const int *sys_n, const floa
               dalloc(&b->pos, N) && dalloc(&b->sorted, N) && dalloc(&b->hist, tiles * nbx::kRadix + nbx::kRadix) &&
               dalloc(&b->child, 8 * M) && dalloc(&b->index, 1) && dalloc(&b->count, M) && dalloc(&b->start, M) &&
               dalloc(&b->cell_level, M) &&
-              dalloc(&b->mass64, M) && dalloc(&b->mom64, 3 * M) && dalloc(&b->nodes4, M) &&
+              dalloc(&b->sum64, M) && dalloc(&b->nodes4, M) &&
               dalloc(&b->fnode, M + 8) && dalloc(&b->fmeta, M + 8) && dalloc(&b->boff, M) &&
               dalloc(&b->bsum, M / nbx::kOffRun + 2) && dalloc(&b->froot4, S) &&
               dalloc(&b->frootm, S) && dalloc(&b->pblk, M) && dalloc(&b->bpar, N) && dalloc(&b->counts, N) && dalloc(&b->acc_scratch, 3 * N) && dalloc(&b->bodies4, N) &&
@@ -166,7 +166,7 @@ int nbx_batch_destroy(nbx_batch *b)
     drop_graph(b);
     void *bufs[] = {b->st, b->off, b->sys_of, b->theta, b->groups, b->ext, b->mm, b->keys, b->skeys, b->kbuf,
                     b->k2, b->k2buf, b->perm, b->perm1, b->vbuf, b->pos, b->sorted, b->hist, b->child, b->index,
-                    b->count, b->start, b->cell_level, b->mass64, b->mom64, b->nodes4,
+                    b->count, b->start, b->cell_level, b->sum64, b->nodes4,
                     b->fnode, b->fmeta, b->boff, b->bsum, b->froot4, b->frootm, b->pblk, b->bpar, b->counts, b->acc_scratch, b->samp_off, b->bodies4, b->gkey, b->gkey2, b->gorder, b->walk, b->gcost,
                     b->bchunk, b->bchunk_off, b->bpart};
     for (void *q : bufs)
@@ -346,23 +346,17 @@ int nbx_batch_copy_buffer(nbx_batch *b, int which, void *dst, int64_t count, voi
     case NBX_BUF_PERM: src = b->perm; elem = 4; cap = b->N; break;
     case NBX_BUF_EXTENT: src = b->ext; elem = 8; cap = 4 * (size_t)b->S; break;
     case NBX_BUF_CELL_LEVEL: src = b->cell_level; elem = 4; cap = b->m; break;
-    case NBX_BUF_CELL_MASS64: src = b->mass64; elem = 8; cap = b->m; break;
-    case NBX_BUF_CELL_COM64: src = b->mom64; elem = 8; cap = 3 * (size_t)b->m; break;
+    case NBX_BUF_CELL_MASS64: elem = 8; cap = b->m; break;
+    case NBX_BUF_CELL_COM64: elem = 8; cap = 3 * (size_t)b->m; break;
     default: return bfail(b, NBX_ERR_INVALID, "unknown buffer");
     }
     if ((size_t)count > cap) return bfail(b, NBX_ERR_INVALID, "count exceeds buffer");
-    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess ||
-        hipMemcpy(dst, src, elem * (size_t)count, hipMemcpyDeviceToHost) != hipSuccess)
-        return bfail(b, NBX_ERR_HIP, "copy failed");
-    if (which == NBX_BUF_CELL_COM64) {
-        size_t nc = (size_t)count / 3;
-        std::vector<double> ms(nc ? nc : 1);
-        if (hipMemcpy(ms.data(), b->mass64, 8 * nc, hipMemcpyDeviceToHost) != hipSuccess)
-            return bfail(b, NBX_ERR_HIP, "copy failed");
-        double *d = static_cast<double *>(dst);
-        for (size_t k = 0; k < nc; ++k)
-            for (int a = 0; a < 3; ++a) d[3 * k + a] = ms[k] != 0.0 ? d[3 * k + a] / ms[k] : 0.0;
-    }
+    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return bfail(b, NBX_ERR_HIP, "copy failed");
+    const hipError_t e = which == NBX_BUF_CELL_MASS64 || which == NBX_BUF_CELL_COM64
+                             ? nbx::copy_cell_sums(b->sum64, which == NBX_BUF_CELL_COM64, static_cast<double *>(dst),
+                                                   (size_t)count)
+                             : hipMemcpy(dst, src, elem * (size_t)count, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return bfail(b, NBX_ERR_HIP, "copy failed");
     return NBX_OK;
 }
 

@@ -113,6 +113,11 @@ struct TreeScratch {
     int samples_max = 0, sub_max = 0;
 };
 
+// Host copy of the FP64 cell sums as NBX_BUF_CELL_MASS64 (count masses) or, com,
+// NBX_BUF_CELL_COM64 (count coordinates, three per cell: the moments divided by
+// the mass, 0 for a massless cell).
+hipError_t copy_cell_sums(const double4 *sum64, bool com, double *dst, size_t count);
+
 // Message returned by nbx_last_error(NULL) (errors of calls without a context).
 void set_create_error(const std::string &msg);
 
@@ -138,7 +143,7 @@ struct nbx_context {
     uint32_t *hist = nullptr;
     int hist_tiles = 0;
     int32_t *cell_level = nullptr;
-    double *mass64 = nullptr, *mom64 = nullptr;
+    double4 *sum64 = nullptr;    // [m_max] FP64 sums of each cell: mass, mass moments x, y, z
     float4 *nodes4 = nullptr;
     float4 *fnode = nullptr;     // [m_max + 8] frame records of the force walk: child (x, y, z, m)
     int2 *fmeta = nullptr;       // [m_max + 8] child descriptors
@@ -212,7 +217,7 @@ struct nbx_batch {
     uint32_t *hist = nullptr;
     int *child = nullptr, *index = nullptr, *count = nullptr, *start = nullptr;
     int32_t *cell_level = nullptr;
-    double *mass64 = nullptr, *mom64 = nullptr;
+    double4 *sum64 = nullptr;    // [M] FP64 sums of each cell: mass, mass moments x, y, z
     float4 *nodes4 = nullptr;
     float4 *fnode = nullptr;     // [M + 8] frame records
     int2 *fmeta = nullptr;
