@@ -662,11 +662,13 @@ struct CellSum {
 // subtrees (more would cost occupancy: 4 needs 90 registers against 59), 4 for
 // the top cells.  CURSOR: the build has given every cell its first sorted body
 // (first >= 0), which the summary then neither reads for the children nor
-// stores again.
-template <int B, bool CURSOR>
+// stores again.  LCNT: the counts of the cells [lbase, ...) of a subtree are
+// in LDS at lcnt (every cell child of a subtree cell is one of them).
+template <int B, bool CURSOR, bool LCNT>
 __device__ inline CellSum sum_cell(int c, int first, bool bucket, int n, int *child, const int *lslots,
                                    const float4 *bodies4, const float4 *sorted4, const int32_t *pos,
-                                   const double4 *sum64, const int *count, const int *start)
+                                   const double4 *sum64, const int *count, const int *start, const int *lcnt,
+                                   int lbase)
 {
     int v[8];
     const int *src = lslots ? lslots : child + 8 * (int64_t)c;
@@ -695,7 +697,7 @@ __device__ inline CellSum sum_cell(int c, int first, bool bucket, int n, int *ch
             cx[t] = cs.y;
             cy[t] = cs.z;
             cz[t] = cs.w;
-            kc[t] = count[q];
+            kc[t] = LCNT ? (v[s0 + t] >= n ? lcnt[v[s0 + t] - lbase] : 0) : count[q];
             kf[t] = CURSOR ? 0 : start[q];
         }
 #pragma unroll
@@ -751,16 +753,17 @@ struct SumOut {
 };
 
 // lev: the cell's level when the caller has it (else -1); lslots: an LDS copy of
-// its slots (else nullptr).
+// its slots (else nullptr); with LCNT, lcnt holds the subtree's counts in LDS
+// from lbase and gets this cell's count too.
 // B: slots whose loads go together; CURSOR: o.cursor (sum_cell).
-template <int B, bool CURSOR>
+template <int B, bool CURSOR, bool LCNT = false>
 __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool single_root, int root_first = 0,
-                                 int lev = -1, const int *lslots = nullptr)
+                                 int lev = -1, const int *lslots = nullptr, int *lcnt = nullptr, int lbase = 0)
 {
     const bool bucket = !single_root && (lev >= 0 ? lev : o.cell_level[c]) == kKeyLevels;
     const int first = !CURSOR ? -1 : single_root ? root_first : o.start[c];
-    const CellSum r = sum_cell<B, CURSOR>(c, first, bucket, o.n, o.child, lslots, o.bodies4, o.nodes4, o.pos, o.sum64,
-                                  o.count, o.start);
+    const CellSum r = sum_cell<B, CURSOR, LCNT>(c, first, bucket, o.n, o.child, lslots, o.bodies4, o.nodes4, o.pos, o.sum64,
+                                  o.count, o.start, lcnt, lbase);
     if (r.cnt > 0 && !(r.ms > 0.0)) atomicOr(&st->err, kErrNonFinite);
     const double gx = r.mx / r.ms, gy = r.my / r.ms, gz = r.mz / r.ms;
     const float fx = (float)gx, fy = (float)gy, fz = (float)gz, fm = (float)r.ms;
@@ -774,6 +777,7 @@ __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool s
         o.cx[c] = fx; o.cy[c] = fy; o.cz[c] = fz; o.mass[c] = fm;
     }
     o.count[c] = r.cnt;
+    if (LCNT) lcnt[c - lbase] = r.cnt;
     if (!CURSOR) o.start[c] = r.first;
     o.nodes4[c] = make_float4(fx, fy, fz, fm);
 }
@@ -782,7 +786,8 @@ __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool s
 // cells of a subtree are the contiguous ids [base, base + ks), summed deepest
 // level first with a barrier between levels.  A subtree of up to kSumLdsCells
 // cells first copies its cells' levels and slots into LDS (one coalesced read),
-// so a level costs the reads of the children's sums and the stores only.
+// and keeps the count of every cell it has summed there, so a level costs the
+// reads of the children's FP64 sums and bodies and the stores only.
 constexpr int kSumLdsCells = 512;
 constexpr int kSubSumBatch = 2;
 template <bool CURSOR>
@@ -792,6 +797,7 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
     __shared__ int s_job;
     __shared__ int s_lev[kSumLdsCells];
     __shared__ int s_slot[8 * kSumLdsCells];
+    __shared__ int s_cnt[kSumLdsCells];  // counts of the subtree's summed cells
     const int tid = threadIdx.x;
     const int nsub = __hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
@@ -815,13 +821,20 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
             for (int k = tid; k < 8 * ks; k += blockDim.x) s_slot[k] = o.child[8 * (int64_t)base + k];
             __syncthreads();
         }
-        for (int L = sc.w; L >= L0; --L) {
-            for (int k = tid; k < ks; k += blockDim.x) {
-                const int lev = lds ? s_lev[k] : o.cell_level[base + k];
-                if (lev == L)
-                    summarise<kSubSumBatch, CURSOR>(o, base + k, st, false, 0, lev, lds ? s_slot + 8 * k : nullptr);
+        if (lds) {  // levels, slots and the counts of summed cells in LDS
+            for (int L = sc.w; L >= L0; --L) {
+                for (int k = tid; k < ks; k += blockDim.x)
+                    if (s_lev[k] == L)
+                        summarise<kSubSumBatch, CURSOR, true>(o, base + k, st, false, 0, L, s_slot + 8 * k, s_cnt,
+                                                              base);
+                __syncthreads();
             }
-            __syncthreads();
+        } else {
+            for (int L = sc.w; L >= L0; --L) {
+                for (int k = tid; k < ks; k += blockDim.x)
+                    if (o.cell_level[base + k] == L) summarise<kSubSumBatch, CURSOR>(o, base + k, st, false, 0, L);
+                __syncthreads();
+            }
         }
     }
 }
