@@ -23,7 +23,7 @@
 //      at workgroup scope; subtree_insert_kernel) builds the same cells when
 //      nbx_params.build_locked is set;
 //   3. the summary runs level by level: subtrees inside their workgroup, then
-//      the top cells in one block.  Every cell is summed by one thread over its
+//      the top cells.  Every cell is summed by one thread over its
 //      children in slot order, in FP64: results do not depend on scheduling.
 #include "nbx_device.h"
 
@@ -92,7 +92,6 @@ __global__ void __launch_bounds__(256) lcp_kernel(int n, const uint64_t *__restr
         st->work = 0;
         st->work2 = 0;
         st->ntop = 0;
-        for (int L = 0; L < kLevels; ++L) st->topl[L] = 0;
     }
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n - 1; k += blockDim.x * gridDim.x) {
         const uint64_t x = (skeys[k] ^ skeys[k + 1]) & ~(1ull << 63);
@@ -104,7 +103,10 @@ __global__ void __launch_bounds__(256) lcp_kernel(int n, const uint64_t *__restr
 // position of a segment.  The level-L cell through q is [a, b), found by two
 // binary searches over the segment; it is a top cell when it holds more than
 // kSubtreeMax bodies, and q owns it when no earlier sample lies in it.  The
-// levels of a sample are searched side by side, not one after another.
+// levels of a sample are searched side by side, not one after another.  The
+// table is level-major (entry L * nsamples + k), so the scan of its flags
+// numbers the top cells level by level: the cells of level L are the ids
+// base0 + [id[L * nsamples], id[(L + 1) * nsamples]).
 __global__ void __launch_bounds__(256) top_cells_kernel(Segs g, int nsamples, const uint64_t *__restrict__ skeys,
                                                         uint32_t *__restrict__ flag, uint32_t *__restrict__ id,
                                                         int32_t *__restrict__ ta, int32_t *__restrict__ tb,
@@ -114,8 +116,9 @@ __global__ void __launch_bounds__(256) top_cells_kernel(Segs g, int nsamples, co
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = gt >> 5, L = gt & 31;
     if (k >= nsamples || L >= kLevels) return;
-    flag[k * kLevels + L] = 0u;
-    id[k * kLevels + L] = 0u;
+    const int e = L * nsamples + k;
+    flag[e] = 0u;
+    id[e] = 0u;
     if (L == 0) return;  // the root is wired separately
     const int s = seg_of_sample(g, k);
     const int lo = seg_lo(g, s), hi = seg_hi(g, s);
@@ -124,10 +127,10 @@ __global__ void __launch_bounds__(256) top_cells_kernel(Segs g, int nsamples, co
     const int a = lower_prefix(skeys, lo, q + 1, P, L);
     const int b = upper_prefix(skeys, q, hi, P, L);
     if (b - a > kSubtreeMax && (q - kSubtreeMax < a || q == lo)) {
-        flag[k * kLevels + L] = 1u;
-        id[k * kLevels + L] = 1u;
-        ta[k * kLevels + L] = a;
-        tb[k * kLevels + L] = b;
+        flag[e] = 1u;
+        id[e] = 1u;
+        ta[e] = a;
+        tb[e] = b;
     }
 }
 
@@ -141,12 +144,12 @@ __global__ void __launch_bounds__(256) top_wire_kernel(
     Segs g, int nsamples, const uint64_t *__restrict__ skeys, const int32_t *__restrict__ perm, int *child,
     int32_t *__restrict__ cell_level, const uint32_t *__restrict__ flag, const uint32_t *__restrict__ id,
     const int32_t *__restrict__ ta, const int32_t *__restrict__ tb, int4 *__restrict__ sub, int sub_max,
-    int32_t *__restrict__ top_list, int base0, int m, int *index, int *__restrict__ start, DeviceState *st)
+    int base0, int m, int *index, int *__restrict__ start, DeviceState *st)
 {
     if (latched(st)) return;
     const int total = nsamples * kLevels;
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int t = gt >> 3;  // entry: one (sample, level) of the top table, or one root
+    const int t = gt >> 3;  // entry: one (level, sample) of the top table, or one root
     const int d = gt & 7;   // digit, slot
     if (gt == 0) {
         const int ntop = (int)(id[total - 1] + flag[total - 1]);
@@ -157,8 +160,8 @@ __global__ void __launch_bounds__(256) top_wire_kernel(
     int c, L, a, b, s;
     if (t < total) {
         if (!flag[t]) return;
-        const int k = t / kLevels;
-        L = t - k * kLevels;
+        L = t / nsamples;
+        const int k = t - L * nsamples;
         c = base0 + (int)id[t];
         a = ta[t];
         b = tb[t];
@@ -167,7 +170,6 @@ __global__ void __launch_bounds__(256) top_wire_kernel(
         if (d == 0) {
             cell_level[c] = L;
             start[c] = a;
-            top_list[L * nsamples + atomicAdd(&st->topl[L], 1)] = c;
         }
         if (L == kKeyLevels) {  // more than kSubtreeMax bodies on one key
             if (d == 0) atomicOr(&st->err, kErrBucket);
@@ -197,7 +199,7 @@ __global__ void __launch_bounds__(256) top_wire_kernel(
         child[slot] = perm[da];
     } else if (cnt > kSubtreeMax) {
         const int kk = samp_lo(g, s) + (da - lo + kSubtreeMax - 1) / kSubtreeMax;
-        child[slot] = base0 + (int)id[kk * kLevels + L + 1];
+        child[slot] = base0 + (int)id[(L + 1) * nsamples + kk];
     } else {
         child[slot] = -1;  // set by the subtree build
         const int i = atomicAdd(&st->nsub, 1);
@@ -843,8 +845,8 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
 // level's cells read only the level below.  A cell's sum gathers from up to 8
 // children in unrelated cache lines, and one CU's address pipeline takes such
 // gathers a few lanes at a time, so a level of more than kTopNarrow cells is
-// spread over kTopBlocks workgroups (cell i to workgroup i mod kTopBlocks) that
-// meet at a grid barrier after it; the narrower levels above it (a top cell's
+// spread over kTopBlocks workgroups (its cell i to workgroup i mod kTopBlocks)
+// that meet at a grid barrier after it; the narrower levels above it (a top cell's
 // parent is a top cell: a level never has more cells than the one below) and
 // the roots are summed by workgroup 0 alone, with workgroup barriers.
 // kTopBlocks workgroups of 256 threads fit on the device at once, so every
@@ -878,17 +880,21 @@ __device__ inline void top_barrier(DeviceState *st, int phase, bool &stalled)
 }
 
 template <bool CURSOR>
-__global__ void __launch_bounds__(256) top_summary_kernel(SumOut o, Segs g, int nsamples,
-                                                          const int32_t *__restrict__ top_list, DeviceState *st)
+__global__ void __launch_bounds__(256) top_summary_kernel(SumOut o, Segs g, int base0, int nsamples,
+                                                          const uint32_t *__restrict__ top_id, DeviceState *st)
 {
-    __shared__ int s_cnt[kLevels];
+    __shared__ int s_first[kLevels + 1];
     __shared__ int s_stop;
     const int tid = threadIdx.x;
     // The subtree summary before this launch is done with its job counter: set
     // it back for a later summary of the same tree.
     if (blockIdx.x == 0 && tid == 0) st->work2 = 0;
     if (tid == 0) s_stop = latched(st);  // one read for the whole workgroup
-    if (tid < kLevels) s_cnt[tid] = st->topl[tid];  // every level's count, read once
+    // The top cells of level L are the ids base0 + [s_first[L], s_first[L + 1])
+    // (the scanned, level-major top table); every workgroup reads the same.
+    if (tid <= kLevels)
+        s_first[tid] = tid < kLevels ? (int)top_id[tid * nsamples]
+                                     : __hip_atomic_load(&st->ntop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     // Latched before this launch: every workgroup reads it so and skips the sums,
     // but still meets every barrier (a latch raised by this launch's own sums may
@@ -897,19 +903,19 @@ __global__ void __launch_bounds__(256) top_summary_kernel(SumOut o, Segs g, int 
     bool stalled = false;
     int L = kKeyLevels, phase = 0;
     for (; L >= 1; --L) {
-        const int cnt = s_cnt[L];
-        if (cnt == 0) continue;  // uniform: no barrier for an empty level
+        const int cnt = s_first[L + 1] - s_first[L];
+        if (cnt <= 0) continue;  // uniform: no barrier for an empty level
         if (cnt <= kTopNarrow) break;
         if (!stop)
             for (int i = blockIdx.x + gridDim.x * tid; i < cnt; i += gridDim.x * blockDim.x)
-                summarise<4, CURSOR>(o, top_list[L * nsamples + i], st, false);
+                summarise<4, CURSOR>(o, base0 + s_first[L] + i, st, false, 0, L);
         top_barrier(st, ++phase, stalled);
     }
     if (blockIdx.x == 0 && !stop) {
         for (; L >= 1; --L) {
-            const int cnt = s_cnt[L];
-            if (cnt == 0) continue;
-            for (int i = tid; i < cnt; i += blockDim.x) summarise<4, CURSOR>(o, top_list[L * nsamples + i], st, false);
+            const int cnt = s_first[L + 1] - s_first[L];
+            if (cnt <= 0) continue;
+            for (int i = tid; i < cnt; i += blockDim.x) summarise<4, CURSOR>(o, base0 + s_first[L] + i, st, false, 0, L);
             __syncthreads();
         }
         for (int s = tid; s < g.nseg; s += blockDim.x) summarise<4, CURSOR>(o, seg_root(g, s), st, !g.batch);
@@ -1216,7 +1222,7 @@ static void build_tree(Segs g, int nsamples, int base0, int m, TreeScratch &ts, 
     launch_exclusive_scan_u32(ts.top_id, nsamples * kLevels, ts.scan_tmp, s);
     const int wire = nsamples * kLevels + g.nseg;  // entries, 8 lanes each
     top_wire_kernel<<<(8 * wire + 255) / 256, 256, 0, s>>>(g, nsamples, skeys, perm, child, cell_level, ts.top_flag,
-                                                        ts.top_id, ts.top_a, ts.top_b, ts.sub, ts.sub_max, ts.top_list,
+                                                        ts.top_id, ts.top_a, ts.top_b, ts.sub, ts.sub_max,
                                                         base0, m, index, start, st);
     subtree_cells_kernel<<<capped_grid(64 * (int64_t)ts.sub_max, 4096), 256, 0, s>>>(ts.sub, ts.sub_max, ts.lcp,
                                                                                      ts.cbase, st);
@@ -1230,15 +1236,16 @@ static void build_tree(Segs g, int nsamples, int base0, int m, TreeScratch &ts, 
                                                          start, ts.sub, ts.sub_max, ts.subcells, st);
 }
 
-static void summarise_tree(Segs g, int nsamples, TreeScratch &ts, DeviceState *st, hipStream_t s,
+// base0: the first top cell's id; nsamples: the top table's samples.
+static void summarise_tree(Segs g, int base0, int nsamples, TreeScratch &ts, DeviceState *st, hipStream_t s,
                            const SumOut &o)
 {
     if (o.cursor) {
         subtree_summary_kernel<true><<<kPersistentBlocks, 256, 0, s>>>(o, ts.subcells, st);
-        top_summary_kernel<true><<<kTopBlocks, 256, 0, s>>>(o, g, nsamples, ts.top_list, st);
+        top_summary_kernel<true><<<kTopBlocks, 256, 0, s>>>(o, g, base0, nsamples, ts.top_id, st);
     } else {
         subtree_summary_kernel<false><<<kPersistentBlocks, 256, 0, s>>>(o, ts.subcells, st);
-        top_summary_kernel<false><<<kTopBlocks, 256, 0, s>>>(o, g, nsamples, ts.top_list, st);
+        top_summary_kernel<false><<<kTopBlocks, 256, 0, s>>>(o, g, base0, nsamples, ts.top_id, st);
     }
 }
 
@@ -1259,13 +1266,12 @@ bool alloc_tree_scratch(TreeScratch &ts, int n, int nseg, int samples)
            hipMalloc((void **)&ts.sub, sizeof(int4) * (size_t)ts.sub_max) == hipSuccess &&
            hipMalloc((void **)&ts.subcells, sizeof(int4) * (size_t)ts.sub_max) == hipSuccess &&
            hipMalloc((void **)&ts.cbase, sizeof(uint32_t) * ((size_t)ts.sub_max + 1)) == hipSuccess &&
-           hipMalloc((void **)&ts.top_list, 4 * tt) == hipSuccess &&
            hipMalloc((void **)&ts.scan_tmp, 4 * 1024) == hipSuccess && (int64_t)samples * kLevels <= (1 << 20);
 }
 
 void free_tree_scratch(TreeScratch &ts)
 {
-    void *p[] = {ts.lcp, ts.top_flag, ts.top_id, ts.top_a, ts.top_b, ts.sub, ts.subcells, ts.scan_tmp, ts.top_list,
+    void *p[] = {ts.lcp, ts.top_flag, ts.top_id, ts.top_a, ts.top_b, ts.sub, ts.subcells, ts.scan_tmp,
                  ts.cbase};
     for (void *q : p)
         if (q) hipFree(q);
@@ -1286,7 +1292,7 @@ void launch_summarize(nbx_context *c, hipStream_t s)
     Segs g{nullptr, nullptr, 1, c->n, 0};
     SumOut o{c->n, c->child, c->bodies4, c->mass, c->x, c->y, c->z, c->count, c->start, c->pos,
              c->cell_level, c->sum64, c->nodes4, c->p.build_locked == 0};
-    summarise_tree(g, samples_of(c->n), c->ts, c->st, s, o);
+    summarise_tree(g, c->n, samples_of(c->n), c->ts, c->st, s, o);
 }
 
 void launch_layout(nbx_context *c, hipStream_t s, int *sorted)
@@ -1326,7 +1332,7 @@ void launch_batch_summarize(nbx_batch *b, hipStream_t s)
     Segs g{b->off, b->samp_off, b->S, b->N, 1};
     SumOut o{b->N, b->child, b->bodies4, b->mass, nullptr, nullptr, nullptr, b->count, b->start, b->pos,
              b->cell_level, b->sum64, b->nodes4, b->p.build_locked == 0};
-    summarise_tree(g, b->ts.samples_max, b->ts, b->st, s, o);
+    summarise_tree(g, b->N + b->S, b->ts.samples_max, b->ts, b->st, s, o);
 }
 
 void launch_batch_layout(nbx_batch *b, hipStream_t s)

@@ -81,8 +81,7 @@ struct DeviceState {
     int sticky;         // unreported bits of earlier steps, moved here by the reset stage
     int seen;           // bits of err already reported by the host check (not moved to sticky)
     int nsub;           // subtrees found by the top-tree pass
-    int ntop;           // top cells (excluding roots)
-    int topl[kLevels];  // top cells per level (lists in TreeScratch::top_list)
+    int ntop;           // top cells (excluding roots): ids [base0, base0 + ntop)
     int log_next;       // adjoint: next free record of the node-term log
     int log_count;      // adjoint: records written (sort length)
     // The job counters of the subtree kernels take an atomic from every
@@ -108,7 +107,6 @@ struct TreeScratch {
     int4 *sub = nullptr;         // {first, end, level, slot address}
     int4 *subcells = nullptr;    // {first cell id, cells, level, deepest level} of each subtree
     uint32_t *cbase = nullptr;   // [sub_max + 1] cells of each subtree, then their exclusive sum
-    int32_t *top_list = nullptr; // [kLevels * samples] top cells of each level
     uint32_t *scan_tmp = nullptr;  // 1024 block sums of the top-table scan
     int samples_max = 0, sub_max = 0;
 };
