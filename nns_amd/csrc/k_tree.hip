@@ -669,8 +669,8 @@ struct CellSum {
 template <int B, bool CURSOR, bool LCNT>
 __device__ inline CellSum sum_cell(int c, int first, bool bucket, int n, int *child, const int *lslots,
                                    const float4 *bodies4, const float4 *sorted4, const int32_t *pos,
-                                   const double4 *sum64, const int *count, const int *start, const int *lcnt,
-                                   int lbase)
+                                   const double4 *sum64, const int *count, const int *start,
+                                   const uint16_t *lcnt, int lbase)
 {
     int v[8];
     const int *src = lslots ? lslots : child + 8 * (int64_t)c;
@@ -760,7 +760,7 @@ struct SumOut {
 // B: slots whose loads go together; CURSOR: o.cursor (sum_cell).
 template <int B, bool CURSOR, bool LCNT = false>
 __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool single_root, int root_first = 0,
-                                 int lev = -1, const int *lslots = nullptr, int *lcnt = nullptr, int lbase = 0)
+                                 int lev = -1, const int *lslots = nullptr, uint16_t *lcnt = nullptr, int lbase = 0)
 {
     const bool bucket = !single_root && (lev >= 0 ? lev : o.cell_level[c]) == kKeyLevels;
     const int first = !CURSOR ? -1 : single_root ? root_first : o.start[c];
@@ -779,7 +779,7 @@ __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool s
         o.cx[c] = fx; o.cy[c] = fy; o.cz[c] = fz; o.mass[c] = fm;
     }
     o.count[c] = r.cnt;
-    if (LCNT) lcnt[c - lbase] = r.cnt;
+    if (LCNT) lcnt[c - lbase] = (uint16_t)r.cnt;  // a subtree holds at most kSubtreeMax bodies
     if (!CURSOR) o.start[c] = r.first;
     o.nodes4[c] = make_float4(fx, fy, fz, fm);
 }
@@ -797,9 +797,11 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
                                                               DeviceState *st)
 {
     __shared__ int s_job;
-    __shared__ int s_lev[kSumLdsCells];
+    // 17.9 KB of LDS: eight workgroups of 256 threads fit a CU's 160 KB
+    __shared__ uint8_t s_lev[kSumLdsCells];
     __shared__ int s_slot[8 * kSumLdsCells];
-    __shared__ int s_cnt[kSumLdsCells];  // counts of the subtree's summed cells
+    __shared__ uint16_t s_cnt[kSumLdsCells];  // counts of the subtree's summed cells
+    static_assert(kSubtreeMax < 65536 && kKeyLevels < 256, "counts and levels fit the LDS types");
     const int tid = threadIdx.x;
     const int nsub = __hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
@@ -819,7 +821,7 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
         const int base = sc.x, ks = sc.y, L0 = sc.z;
         const bool lds = ks <= kSumLdsCells;
         if (lds) {
-            for (int k = tid; k < ks; k += blockDim.x) s_lev[k] = o.cell_level[base + k];
+            for (int k = tid; k < ks; k += blockDim.x) s_lev[k] = (uint8_t)o.cell_level[base + k];
             for (int k = tid; k < 8 * ks; k += blockDim.x) s_slot[k] = o.child[8 * (int64_t)base + k];
             __syncthreads();
         }
