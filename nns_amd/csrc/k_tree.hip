@@ -756,14 +756,16 @@ struct SumOut {
 
 // lev: the cell's level when the caller has it (else -1); lslots: an LDS copy of
 // its slots (else nullptr); with LCNT, lcnt holds the subtree's counts in LDS
-// from lbase and gets this cell's count too.
+// from lbase and gets this cell's count too, and lfirst is the cell's first
+// sorted body (staged by the caller; with CURSOR only).
 // B: slots whose loads go together; CURSOR: o.cursor (sum_cell).
 template <int B, bool CURSOR, bool LCNT = false>
 __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool single_root, int root_first = 0,
-                                 int lev = -1, const int *lslots = nullptr, uint16_t *lcnt = nullptr, int lbase = 0)
+                                 int lev = -1, const int *lslots = nullptr, uint16_t *lcnt = nullptr, int lbase = 0,
+                                 int lfirst = -1)
 {
     const bool bucket = !single_root && (lev >= 0 ? lev : o.cell_level[c]) == kKeyLevels;
-    const int first = !CURSOR ? -1 : single_root ? root_first : o.start[c];
+    const int first = !CURSOR ? -1 : single_root ? root_first : LCNT ? lfirst : o.start[c];
     const CellSum r = sum_cell<B, CURSOR, LCNT>(c, first, bucket, o.n, o.child, lslots, o.bodies4, o.nodes4, o.pos, o.sum64,
                                   o.count, o.start, lcnt, lbase);
     if (r.cnt > 0 && !(r.ms > 0.0)) atomicOr(&st->err, kErrNonFinite);
@@ -797,10 +799,12 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
                                                               DeviceState *st)
 {
     __shared__ int s_job;
-    // 17.9 KB of LDS: eight workgroups of 256 threads fit a CU's 160 KB
+    // 19,972 bytes of LDS (20 KB as allocated): eight workgroups of 256 threads
+    // fit a CU's 160 KB
     __shared__ uint8_t s_lev[kSumLdsCells];
     __shared__ int s_slot[8 * kSumLdsCells];
     __shared__ uint16_t s_cnt[kSumLdsCells];  // counts of the subtree's summed cells
+    __shared__ int s_first[kSumLdsCells];     // first sorted body of each cell (CURSOR)
     static_assert(kSubtreeMax < 65536 && kKeyLevels < 256, "counts and levels fit the LDS types");
     const int tid = threadIdx.x;
     const int nsub = __hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -821,7 +825,10 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
         const int base = sc.x, ks = sc.y, L0 = sc.z;
         const bool lds = ks <= kSumLdsCells;
         if (lds) {
-            for (int k = tid; k < ks; k += blockDim.x) s_lev[k] = (uint8_t)o.cell_level[base + k];
+            for (int k = tid; k < ks; k += blockDim.x) {
+                s_lev[k] = (uint8_t)o.cell_level[base + k];
+                if (CURSOR) s_first[k] = o.start[base + k];
+            }
             for (int k = tid; k < 8 * ks; k += blockDim.x) s_slot[k] = o.child[8 * (int64_t)base + k];
             __syncthreads();
         }
@@ -830,7 +837,7 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
                 for (int k = tid; k < ks; k += blockDim.x)
                     if (s_lev[k] == L)
                         summarise<kSubSumBatch, CURSOR, true>(o, base + k, st, false, 0, L, s_slot + 8 * k, s_cnt,
-                                                              base);
+                                                              base, CURSOR ? s_first[k] : -1);
                 __syncthreads();
             }
         } else {
