@@ -100,8 +100,8 @@ void nbx_default_params(nbx_params *p);
  * m = 2n + 12000, src/BarnesHut.cu:18).  device: HIP device ordinal the context
  * allocates on and launches on.  n_max is at most 48,805,888 (NBX_ERR_INVALID
  * above): the tree's top table holds 2^20 entries, 22 per 1024 bodies.  m_max is
- * at most 2^27 = 134,217,728: the force walk's node descriptors hold a node's
- * block index times 16 in 32 bits. */
+ * at most 2^27 = 134,217,728: the force walk's node descriptors hold a frame
+ * record offset (or a sorted position) times 16 in 32 bits. */
 int nbx_create(const nbx_params *p, int n_max, int m_max, int device, nbx_context **out);
 int nbx_destroy(nbx_context *ctx);
 int nbx_set_params(nbx_context *ctx, const nbx_params *p);
@@ -263,7 +263,7 @@ int nbx_autotune(nbx_context *ctx, void *stream, int *waves_out);
  * flow down to the bodies through the centre-of-mass sums (FP64).  The node terms
  * are logged, sorted by node and summed; the order of records within a node
  * follows the log, so the last bits may differ between runs.  The first call
- * allocates the scratch: 64 (m_max + 1) bytes plus 40 bytes per log record, with
+ * allocates the scratch: 64 (n_max + m_max + 8) bytes plus 40 bytes per log record, with
  * 64 n_max + 4 n_max records (about 2.6 KB per body); if a call needs more
  * records, the rest are added with atomics. */
 int nbx_force_vjp(nbx_context *ctx, const float *ux, const float *uy, const float *uz, float g, float *gx,
