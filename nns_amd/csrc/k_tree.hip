@@ -90,7 +90,6 @@ __global__ void __launch_bounds__(256) lcp_kernel(int n, const uint64_t *__restr
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->nsub = 0;
         st->work = 0;
-        st->work2 = 0;
         st->ntop = 0;
     }
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n - 1; k += blockDim.x * gridDim.x) {
@@ -485,7 +484,11 @@ __global__ void __launch_bounds__(256) subtree_insert_kernel(
 // to -1, waits for those stores, and every cell and body stores itself into
 // its parent.  Every id and slot is range-checked before its store.
 constexpr int kLiftLevels = 10;  // 2^kLiftLevels >= kSubtreeMax
-constexpr int kBuildBlocks = 4096;
+// Workgroups stride over the jobs.  With about one job each at N = 4M (16,400
+// subtrees), the dispatcher starts a new workgroup wherever one has finished,
+// which balances subtrees of 2 to 1024 bodies better than a few jobs per
+// resident workgroup (0.141 ms with 4096 workgroups, 0.103 ms with 16384).
+constexpr int kBuildBlocks = 16384;
 
 __device__ inline int wave_exclusive_sum(int v, int *total)
 {
@@ -786,13 +789,16 @@ __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool s
     o.nodes4[c] = make_float4(fx, fy, fz, fm);
 }
 
-// Subtree summary: persistent workgroups take subtrees from the queue; the
-// cells of a subtree are the contiguous ids [base, base + ks), summed deepest
+// Subtree summary: workgroup w sums subtrees w, w + gridDim.x, ...: with as many
+// workgroups as subtrees, about, the dispatcher hands each free slot of a CU
+// the next workgroup, a job queue with no shared counter.  The cells of a
+// subtree are the contiguous ids [base, base + ks), summed deepest
 // level first with a barrier between levels.  A subtree of up to kSumLdsCells
 // cells first copies its cells' levels and slots into LDS (one coalesced read),
 // and keeps the count of every cell it has summed there, so a level costs the
 // reads of the children's FP64 sums and bodies and the stores only.
 constexpr int kSumLdsCells = 512;
+constexpr int kSumBlocks = 16384;  // 0.317 ms with 2048 workgroups taking jobs from a counter, 0.308 ms so
 constexpr int kSubSumBatch = 2;
 template <bool CURSOR>
 __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const int4 *__restrict__ subcells,
@@ -808,15 +814,13 @@ __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const in
     static_assert(kSubtreeMax < 65536 && kKeyLevels < 256, "counts and levels fit the LDS types");
     const int tid = threadIdx.x;
     const int nsub = __hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int next = blockIdx.x;
     for (;;) {
         // One read of the latch per job, by thread 0, published with the job:
         // the workgroup continues or leaves as a whole (a zero cell mass raises
-        // the latch from inside this launch).  Both requests are issued at once,
-        // as in subtree_insert_kernel.
-        if (tid == 0) {
-            const int j = atomicAdd(&st->work2, 1);
-            s_job = latched(st) ? nsub : j;
-        }
+        // the latch from inside this launch).
+        if (tid == 0) s_job = latched(st) ? nsub : next;
+        next += gridDim.x;
         __syncthreads();
         const int job = s_job;
         __syncthreads();
@@ -895,9 +899,6 @@ __global__ void __launch_bounds__(256) top_summary_kernel(SumOut o, Segs g, int 
     __shared__ int s_first[kLevels + 1];
     __shared__ int s_stop;
     const int tid = threadIdx.x;
-    // The subtree summary before this launch is done with its job counter: set
-    // it back for a later summary of the same tree.
-    if (blockIdx.x == 0 && tid == 0) st->work2 = 0;
     if (tid == 0) s_stop = latched(st);  // one read for the whole workgroup
     // The top cells of level L are the ids base0 + [s_first[L], s_first[L + 1])
     // (the scanned, level-major top table); every workgroup reads the same.
@@ -1250,10 +1251,10 @@ static void summarise_tree(Segs g, int base0, int nsamples, TreeScratch &ts, Dev
                            const SumOut &o)
 {
     if (o.cursor) {
-        subtree_summary_kernel<true><<<kPersistentBlocks, 256, 0, s>>>(o, ts.subcells, st);
+        subtree_summary_kernel<true><<<kSumBlocks, 256, 0, s>>>(o, ts.subcells, st);
         top_summary_kernel<true><<<kTopBlocks, 256, 0, s>>>(o, g, base0, nsamples, ts.top_id, st);
     } else {
-        subtree_summary_kernel<false><<<kPersistentBlocks, 256, 0, s>>>(o, ts.subcells, st);
+        subtree_summary_kernel<false><<<kSumBlocks, 256, 0, s>>>(o, ts.subcells, st);
         top_summary_kernel<false><<<kTopBlocks, 256, 0, s>>>(o, g, base0, nsamples, ts.top_id, st);
     }
 }

@@ -1,7 +1,8 @@
-"""Host replay of the job loop of the persistent tree kernels (subtree_insert_kernel
-and subtree_summary_kernel in nns_amd/csrc/k_tree.hip): one workgroup of several
-waves takes jobs through an LDS word that only thread 0 (in wave 0) writes, while
-another workgroup of the same launch may raise the error latch at any moment.
+"""Host replay of the job loop of the tree kernels (subtree_insert_kernel, which
+takes jobs from a shared counter, and subtree_summary_kernel, which strides over
+them, in nns_amd/csrc/k_tree.hip): one workgroup of several waves takes jobs
+through an LDS word that only thread 0 (in wave 0) writes, while another
+workgroup of the same launch may raise the error latch at any moment.
 
 Every interleaving of the waves' steps with the moment the latch is raised is
 replayed.  A barrier completes once every wave that has not left has arrived
