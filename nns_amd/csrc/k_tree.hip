@@ -89,7 +89,6 @@ __global__ void __launch_bounds__(256) lcp_kernel(int n, const uint64_t *__restr
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->nsub = 0;
-        st->work = 0;
         st->ntop = 0;
     }
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n - 1; k += blockDim.x * gridDim.x) {
@@ -340,19 +339,15 @@ __global__ void __launch_bounds__(256) subtree_insert_kernel(
     const int tid = threadIdx.x;
     const int nsub = min(__hip_atomic_load(&st->nsub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), sub_max);
     const int top = base0 + __hip_atomic_load(&st->ntop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int next = blockIdx.x;  // workgroups stride over the jobs, like the summary's
     for (;;) {
         // Thread 0 alone reads the error latch (which other workgroups of this
-        // launch may raise at any time) and takes the job; every wave reads its
+        // launch may raise at any time) and names the job; every wave reads its
         // decision from LDS after the barrier, so the workgroup continues or
         // leaves as a whole.  A wave that left on its own read would let the
-        // others pass the barrier and read an s_job nobody wrote.  The job is
-        // taken whatever the latch says, so that the two requests travel
-        // together (one round trip per job); a job taken under the latch is
-        // dropped.
-        if (tid == 0) {
-            const int j = atomicAdd(&st->work, 1);
-            s_job = latched(st) ? nsub : j;
-        }
+        // others pass the barrier and read an s_job nobody wrote.
+        if (tid == 0) s_job = latched(st) ? nsub : next;
+        next += gridDim.x;
         __syncthreads();
         const int job = s_job;
         __syncthreads();
@@ -1220,7 +1215,6 @@ static inline int capped_grid(int64_t work, int cap = 65536)
     return (int)(g > cap ? cap : g);
 }
 
-constexpr int kPersistentBlocks = 2048;  // 256 CUs x 8 workgroups
 
 static void build_tree(Segs g, int nsamples, int base0, int m, TreeScratch &ts, DeviceState *st, hipStream_t s,
                        int n, const uint64_t *skeys, const int32_t *perm, const uint64_t *keys, int *child,
@@ -1238,7 +1232,7 @@ static void build_tree(Segs g, int nsamples, int base0, int m, TreeScratch &ts, 
                                                                                      ts.cbase, st);
     subtree_base_kernel<<<1, 1024, 0, s>>>(ts.cbase, ts.sub_max, base0, m, index, st);
     if (locked)
-        subtree_insert_kernel<<<kPersistentBlocks, 256, 0, s>>>(n, m, skeys, perm, keys, ts.lcp, child, base0,
+        subtree_insert_kernel<<<kSumBlocks, 256, 0, s>>>(n, m, skeys, perm, keys, ts.lcp, child, base0,
                                                                 ts.cbase, cell_level, ts.sub, ts.sub_max,
                                                                 ts.subcells, st);
     else

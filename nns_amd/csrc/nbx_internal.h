@@ -84,11 +84,6 @@ struct DeviceState {
     int ntop;           // top cells (excluding roots): ids [base0, base0 + ntop)
     int log_next;       // adjoint: next free record of the node-term log
     int log_count;      // adjoint: records written (sort length)
-    // The job counter of the locked build takes an atomic from every workgroup
-    // per job; it sits on a cache line of its own, so that the error latch,
-    // which the same workgroups read once per job, is not queued behind those
-    // atomics.
-    alignas(128) int work;  // next subtree to insert (locked build)
     // Grid barrier of the top summary: arrivals of its workgroups and their
     // exits; the last workgroup to leave sets both back to 0.
     alignas(128) int tsync;
