@@ -1,6 +1,6 @@
-"""Host replay of the job loop of the tree kernels (subtree_insert_kernel, which
-takes jobs from a shared counter, and subtree_summary_kernel, which strides over
-them, in nns_amd/csrc/k_tree.hip): one workgroup of several waves takes jobs
+"""Host replay of the job loop of the tree kernels (subtree_insert_kernel and
+subtree_summary_kernel in nns_amd/csrc/k_tree.hip, which stride over their
+jobs; they took them from a shared counter before): one workgroup of several waves takes jobs
 through an LDS word that only thread 0 (in wave 0) writes, while another
 workgroup of the same launch may raise the error latch at any moment.
 
