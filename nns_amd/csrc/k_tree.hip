@@ -1094,9 +1094,10 @@ __device__ inline int2 chain_bottom_from(int c, int4 a, int4 b, int lev, int n, 
     }
 }
 
-// Ensemble: the system of a cell from its first sorted body.
+// Ensemble: the system of a cell from its first sorted body, whose group key
+// (made before the layout) holds the system index above the Hilbert digits.
 struct SysInfo {
-    const int32_t *start, *perm, *sys_of;
+    const uint64_t *gkey;
     const float *theta;
     const double *ext;
 };
@@ -1152,7 +1153,7 @@ __global__ void __launch_bounds__(256) frame_layout_kernel(
         double span;
         float th;
         if (BATCH) {
-            const int sys = item < nroot ? (int)item : si.sys_of[si.perm[si.start[node]]];
+            const int sys = item < nroot ? (int)item : (int)(si.gkey[nstart] >> (3 * kGroupLevels));
             span = si.ext[4 * sys + 3];
             th = si.theta[sys];
         } else {
@@ -1342,7 +1343,8 @@ void launch_batch_summarize(nbx_batch *b, hipStream_t s)
 void launch_batch_layout(nbx_batch *b, hipStream_t s)
 {
     sorted_index_kernel<<<capped_grid(b->N), 256, 0, s>>>(b->N, b->perm, b->sorted, b->st);
-    SysInfo si{b->start, b->perm, b->sys_of, b->theta, b->ext};
+    group_keys_kernel<uint64_t><<<capped_grid(b->N), 256, 0, s>>>(b->N, b->skeys, b->perm, b->sys_of, b->gkey, b->st);
+    SysInfo si{b->gkey, b->theta, b->ext};
     const int runs = (int)(((int64_t)b->m - b->N + kOffRun - 1) / kOffRun);
     frame_count_kernel<true><<<runs, 256, 0, s>>>(b->N, b->index, b->child, b->boff, b->bsum, b->st);
     launch_scan_one_block(b->bsum, runs, s);
@@ -1351,7 +1353,6 @@ void launch_batch_layout(nbx_batch *b, hipStream_t s)
         b->bsum, b->froot4, b->frootm, b->pblk, b->bpar, 0.f, layout_eps2(b->p), si, b->st);
     int sys_bits = 0;
     while ((1 << sys_bits) < b->S) ++sys_bits;
-    group_keys_kernel<uint64_t><<<capped_grid(b->N), 256, 0, s>>>(b->N, b->skeys, b->perm, b->sys_of, b->gkey, b->st);
     launch_radix_passes(b->hist, s, b->gkey, nullptr, b->N, 0, group_passes(sys_bits), b->gkey2, b->gorder, b->kbuf,
                         b->vbuf, nullptr);
 }
