@@ -793,7 +793,6 @@ __device__ inline void summarise(const SumOut &o, int c, DeviceState *st, bool s
 // and keeps the count of every cell it has summed there, so a level costs the
 // reads of the children's FP64 sums and bodies and the stores only.
 constexpr int kSumLdsCells = 512;
-constexpr int kSumBlocks = 16384;  // 0.317 ms with 2048 workgroups taking jobs from a counter, 0.308 ms so
 constexpr int kSubSumBatch = 2;
 template <bool CURSOR>
 __global__ void __launch_bounds__(256) subtree_summary_kernel(SumOut o, const int4 *__restrict__ subcells,
@@ -1217,6 +1216,16 @@ static inline int capped_grid(int64_t work, int cap = 65536)
 }
 
 
+// Workgroups of a job loop (subtree build, locked insertion, subtree summary):
+// about one per subtree (a subtree averages about 256 bodies), at least
+// kBuildBlocks.  With many jobs per workgroup the stride balances them worse:
+// at N = 48.8M, 16384 summary workgroups took 3.50 ms against 3.30 ms for a
+// shared job counter.
+static int job_grid(int nbodies)
+{
+    return std::min(1 << 20, std::max(kBuildBlocks, nbodies / 256));
+}
+
 static void build_tree(Segs g, int nsamples, int base0, int m, TreeScratch &ts, DeviceState *st, hipStream_t s,
                        int n, const uint64_t *skeys, const int32_t *perm, const uint64_t *keys, int *child,
                        int *index, int32_t *cell_level, int *start, bool locked)
@@ -1233,11 +1242,11 @@ static void build_tree(Segs g, int nsamples, int base0, int m, TreeScratch &ts, 
                                                                                      ts.cbase, st);
     subtree_base_kernel<<<1, 1024, 0, s>>>(ts.cbase, ts.sub_max, base0, m, index, st);
     if (locked)
-        subtree_insert_kernel<<<kSumBlocks, 256, 0, s>>>(n, m, skeys, perm, keys, ts.lcp, child, base0,
+        subtree_insert_kernel<<<job_grid(g.n), 256, 0, s>>>(n, m, skeys, perm, keys, ts.lcp, child, base0,
                                                                 ts.cbase, cell_level, ts.sub, ts.sub_max,
                                                                 ts.subcells, st);
     else
-        subtree_build_kernel<<<kBuildBlocks, 64, 0, s>>>(m, skeys, perm, ts.lcp, child, base0, ts.cbase, cell_level,
+        subtree_build_kernel<<<job_grid(g.n), 64, 0, s>>>(m, skeys, perm, ts.lcp, child, base0, ts.cbase, cell_level,
                                                          start, ts.sub, ts.sub_max, ts.subcells, st);
 }
 
@@ -1246,10 +1255,10 @@ static void summarise_tree(Segs g, int base0, int nsamples, TreeScratch &ts, Dev
                            const SumOut &o)
 {
     if (o.cursor) {
-        subtree_summary_kernel<true><<<kSumBlocks, 256, 0, s>>>(o, ts.subcells, st);
+        subtree_summary_kernel<true><<<job_grid(g.n), 256, 0, s>>>(o, ts.subcells, st);
         top_summary_kernel<true><<<kTopBlocks, 256, 0, s>>>(o, g, base0, nsamples, ts.top_id, st);
     } else {
-        subtree_summary_kernel<false><<<kSumBlocks, 256, 0, s>>>(o, ts.subcells, st);
+        subtree_summary_kernel<false><<<job_grid(g.n), 256, 0, s>>>(o, ts.subcells, st);
         top_summary_kernel<false><<<kTopBlocks, 256, 0, s>>>(o, g, base0, nsamples, ts.top_id, st);
     }
 }
